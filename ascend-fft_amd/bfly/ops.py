@@ -1,0 +1,134 @@
+"""Device FFTs on torch tensors through the C ABI (no torch types cross it).
+
+Tensors stay in HBM; the kernels run on torch's current stream of the
+tensor's device.  Layouts are the reference's transform contract:
+  c2c   complex64 [..., n]  or float32 interleaved [..., 2n]
+  rfft  float32 [..., n]       -> float32 half spectrum [..., n+2]
+  irfft float32 [..., n+2], n  -> float32 [..., n]   (scaled 1/n)
+"""
+from __future__ import annotations
+
+import numpy as np
+import torch
+
+from . import _lib
+
+_tables: dict = {}
+
+
+def _stream(device: torch.device) -> int:
+    return torch.cuda.current_stream(device).cuda_stream
+
+
+def supported(kind: int, n: int) -> bool:
+    return bool(_lib.hip().bfly_supported(kind, n))
+
+
+def table(kind: int, n: int, device) -> torch.Tensor:
+    """Device copy of the FP64-generated twiddle table of (kind, n), cached."""
+    device = torch.device(device)
+    key = (device.index, kind, n)
+    t = _tables.get(key)
+    if t is None:
+        lib = _lib.hip()
+        nf = lib.bfly_table_floats(kind, n)
+        if nf == 0:
+            raise _lib.BflyError(_lib.ERR_SHAPE, f"no kernel for kind={kind} n={n}")
+        host = np.empty(nf, np.float32)
+        _lib.check(lib.bfly_table_fill(kind, n, host.ctypes.data), "bfly_table_fill")
+        t = torch.from_numpy(host).to(device)
+        _tables[key] = t
+    return t
+
+
+def workspace(kind: int, n: int, batch: int, device) -> torch.Tensor | None:
+    nbytes = _lib.hip().bfly_workspace_bytes(kind, n, batch)
+    if nbytes == 0:
+        return None
+    return torch.empty(nbytes // 4, dtype=torch.float32, device=device)
+
+
+def _check_device(t: torch.Tensor, name: str) -> None:
+    if not t.is_cuda:
+        raise ValueError(f"{name} must be a device tensor")
+    if not t.is_contiguous():
+        raise ValueError(f"{name} must be contiguous")
+
+
+def c2c_raw(x: torch.Tensor, out: torch.Tensor, n: int, batch: int, sign: int = -1,
+            work: torch.Tensor | None = None) -> None:
+    """bfly_c2c on interleaved float32 buffers (no allocation except tables)."""
+    tab = table(_lib.KIND_C2C, n, x.device)
+    if work is None and _lib.hip().bfly_workspace_bytes(_lib.KIND_C2C, n, batch):
+        work = workspace(_lib.KIND_C2C, n, batch, x.device)
+    rc = _lib.hip().bfly_c2c(x.data_ptr(), out.data_ptr(), n, batch, sign, tab.data_ptr(),
+                             work.data_ptr() if work is not None else None, _stream(x.device))
+    _lib.check(rc, f"bfly_c2c(n={n}, batch={batch}, sign={sign})")
+
+
+def c2c(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
+    """Unnormalised C2C along the last axis (sign -1 forward, +1 inverse)."""
+    _check_device(x, "x")
+    if x.dtype == torch.complex64:
+        n = x.shape[-1]
+        xf = torch.view_as_real(x)
+    elif x.dtype == torch.float32:
+        if x.shape[-1] % 2:
+            raise ValueError("interleaved input needs an even last dimension")
+        n = x.shape[-1] // 2
+        xf = x
+    else:
+        raise TypeError("c2c takes complex64 or interleaved float32")
+    batch = x.numel() // (n * (1 if x.dtype == torch.complex64 else 2))
+    out = torch.empty_like(x)
+    if batch == 0:
+        return out
+    of = torch.view_as_real(out) if x.dtype == torch.complex64 else out
+    c2c_raw(xf, of, n, batch, sign)
+    return out
+
+
+def rfft(x: torch.Tensor) -> torch.Tensor:
+    """R2C along the last axis: float32 [..., n] -> half spectrum [..., n+2]."""
+    _check_device(x, "x")
+    if x.dtype != torch.float32:
+        raise TypeError("rfft takes float32")
+    n = x.shape[-1]
+    batch = x.numel() // n if n else 0
+    out = torch.empty(*x.shape[:-1], n + 2, dtype=torch.float32, device=x.device)
+    if batch == 0:
+        return out
+    tab = table(_lib.KIND_R2C, n, x.device)
+    rc = _lib.hip().bfly_r2c(x.data_ptr(), out.data_ptr(), n, batch, tab.data_ptr(),
+                             _stream(x.device))
+    _lib.check(rc, f"bfly_r2c(n={n}, batch={batch})")
+    return out
+
+
+def irfft(h: torch.Tensor, n: int) -> torch.Tensor:
+    """C2R along the last axis: half spectrum [..., n+2] -> float32 [..., n], 1/n."""
+    _check_device(h, "h")
+    if h.dtype != torch.float32 or h.shape[-1] != n + 2:
+        raise ValueError("irfft takes float32 [..., n+2]")
+    batch = h.numel() // (n + 2)
+    out = torch.empty(*h.shape[:-1], n, dtype=torch.float32, device=h.device)
+    if batch == 0:
+        return out
+    tab = table(_lib.KIND_C2R, n, h.device)
+    rc = _lib.hip().bfly_c2r(h.data_ptr(), out.data_ptr(), n, batch, tab.data_ptr(),
+                             _stream(h.device))
+    _lib.check(rc, f"bfly_c2r(n={n}, batch={batch})")
+    return out
+
+
+def probe_copy(src: torch.Tensor, dst: torch.Tensor) -> None:
+    rc = _lib.hip().bfly_probe_copy(src.data_ptr(), dst.data_ptr(), src.numel() * src.element_size(),
+                                    _stream(src.device))
+    _lib.check(rc, "bfly_probe_copy")
+
+
+def device_facts(device: int = 0) -> str:
+    import ctypes
+    buf = ctypes.create_string_buffer(1024)
+    _lib.check(_lib.hip().bfly_probe_device(device, buf, 1024), "bfly_probe_device")
+    return buf.value.decode()

@@ -1,0 +1,145 @@
+// bfly_capi.cpp -- the exported C ABI of libbfly_hip.so (include/bfly_hip.h)
+// on the HIP runtime: argument guards, runtime wrappers and kernel dispatch.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstdio>
+#include <cstring>
+
+#include "bfly_hip.h"
+#include "bfly_kernels.hpp"
+#include "bfly_plans.hpp"
+#include "bfly_tables.hpp"
+
+namespace {
+
+inline hipStream_t S(void* stream) { return reinterpret_cast<hipStream_t>(stream); }
+
+int launched(hipError_t e) { return e == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH; }
+
+}  // namespace
+
+extern "C" {
+
+int bfly_rt_init(int device) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return BFLY_ERR_NOT_LOADED;
+    if (device < 0 || device >= count) return BFLY_ERR_TABLE;
+    return hipSetDevice(device) == hipSuccess ? BFLY_OK : BFLY_ERR_TABLE;
+}
+
+int bfly_rt_stream_create(void** stream) {
+    if (!stream) return BFLY_ERR_SHAPE;
+    hipStream_t s = nullptr;
+    if (hipStreamCreateWithFlags(&s, hipStreamNonBlocking) != hipSuccess) return BFLY_ERR_ALLOC;
+    *stream = s;
+    return BFLY_OK;
+}
+
+int bfly_rt_stream_destroy(void* stream) {
+    if (!stream) return BFLY_OK;
+    return hipStreamDestroy(S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
+}
+
+int bfly_rt_stream_sync(void* stream) {
+    return hipStreamSynchronize(S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
+}
+
+int bfly_rt_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return BFLY_ERR_SHAPE;
+    *ptr = nullptr;
+    if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) {
+        *ptr = nullptr;
+        return BFLY_ERR_ALLOC;
+    }
+    return BFLY_OK;
+}
+
+int bfly_rt_free(void* ptr) {
+    if (!ptr) return BFLY_OK;
+    return hipFree(ptr) == hipSuccess ? BFLY_OK : BFLY_ERR_ALLOC;
+}
+
+int bfly_rt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return BFLY_ERR_H2D;
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)) != hipSuccess)
+        return BFLY_ERR_H2D;
+    return hipStreamSynchronize(S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_H2D;
+}
+
+int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return BFLY_ERR_D2H;
+    if (hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)) != hipSuccess)
+        return BFLY_ERR_D2H;
+    return hipStreamSynchronize(S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_D2H;
+}
+
+int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+             const float* table, void* work, void* stream) {
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return g;
+    return launched(bfly_kern::c2c(in, out, n, batch, sign, table, work, S(stream)));
+}
+
+int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
+             void* stream) {
+    if (int g = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return g;
+    return launched(bfly_kern::real(BFLY_KIND_R2C, in, out, n, batch, table, S(stream)));
+}
+
+int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
+             void* stream) {
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return g;
+    return launched(bfly_kern::real(BFLY_KIND_C2R, in, out, n, batch, table, S(stream)));
+}
+
+int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,
+                           uint32_t batch, const float* table, void* work, int reps,
+                           void* stream, float* ms) {
+    if (!ms || reps <= 0) return BFLY_ERR_SHAPE;
+    hipEvent_t e0, e1;
+    if (hipEventCreate(&e0) != hipSuccess) return BFLY_ERR_ALLOC;
+    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return BFLY_ERR_ALLOC; }
+    int rc = BFLY_OK;
+    hipEventRecord(e0, S(stream));
+    for (int i = 0; i < reps && rc == BFLY_OK; i++) {
+        if (kind == BFLY_KIND_C2C) rc = bfly_c2c(in, out, n, batch, sign, table, work, stream);
+        else if (kind == BFLY_KIND_R2C) rc = bfly_r2c(in, out, n, batch, table, stream);
+        else if (kind == BFLY_KIND_C2R) rc = bfly_c2r(in, out, n, batch, table, stream);
+        else rc = BFLY_ERR_SHAPE;
+    }
+    hipEventRecord(e1, S(stream));
+    if (hipEventSynchronize(e1) != hipSuccess && rc == BFLY_OK) rc = BFLY_ERR_SYNC;
+    if (rc == BFLY_OK && hipEventElapsedTime(ms, e0, e1) != hipSuccess) rc = BFLY_ERR_SYNC;
+    hipEventDestroy(e0);
+    hipEventDestroy(e1);
+    return rc;
+}
+
+int bfly_probe_copy(const void* src, void* dst, size_t bytes, void* stream) {
+    if (!src || !dst || bytes == 0 || (bytes & 15) || ((uintptr_t)src & 15) || ((uintptr_t)dst & 15))
+        return BFLY_ERR_SHAPE;
+    return launched(bfly_kern::copy(src, dst, bytes, S(stream)));
+}
+
+int bfly_probe_device(int device, char* json, size_t cap) {
+    if (!json || cap == 0) return BFLY_ERR_SHAPE;
+    hipDeviceProp_t p;
+    if (hipGetDeviceProperties(&p, device) != hipSuccess) return BFLY_ERR_NOT_LOADED;
+    int lds_per_cu = 0, mem_clock_khz = 0, bus_width = 0, l2 = 0;
+    hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device);
+    hipDeviceGetAttribute(&mem_clock_khz, hipDeviceAttributeMemoryClockRate, device);
+    hipDeviceGetAttribute(&bus_width, hipDeviceAttributeMemoryBusWidth, device);
+    hipDeviceGetAttribute(&l2, hipDeviceAttributeL2CacheSize, device);
+    const int w = std::snprintf(
+        json, cap,
+        "{\"name\": \"%s\", \"gcn_arch\": \"%s\", \"compute_units\": %d, \"wave_size\": %d, "
+        "\"lds_bytes_per_workgroup\": %zu, \"lds_bytes_per_cu\": %d, \"l2_bytes\": %d, "
+        "\"global_mem_total_bytes\": %zu, \"max_threads_per_block\": %d, "
+        "\"clock_khz\": %d, \"mem_clock_khz\": %d, \"mem_bus_width_bits\": %d}",
+        p.name, p.gcnArchName, p.multiProcessorCount, p.warpSize, (size_t)p.sharedMemPerBlock,
+        lds_per_cu, l2, (size_t)p.totalGlobalMem, p.maxThreadsPerBlock, p.clockRate,
+        mem_clock_khz, bus_width);
+    return (w > 0 && (size_t)w < cap) ? BFLY_OK : BFLY_ERR_SHAPE;
+}
+
+}  // extern "C"

@@ -1,0 +1,222 @@
+// bfly_device.hpp -- device building blocks for the gfx950 FFT kernels.
+//
+// Complex values are float2 (x = re, y = im).  DIR is the exponent sign of the
+// transform: -1 forward (X[k] = sum x[j] e^{-2 pi i jk/n}), +1 unnormalised
+// inverse.  Every twiddle table holds forward values W = e^{-2 pi i m/L}; the
+// inverse conjugates them on use, so one table serves both directions.
+//
+// The Stockham core keeps one transform's working set in LDS between radix
+// passes and in registers inside a pass: each thread owns P points, a pass of
+// radix R runs P/R butterflies per thread, and only the exchanges between
+// passes touch LDS.  The first pass reads its operands straight from the
+// caller (global memory or a staging buffer) and the last pass leaves its
+// results in registers in natural order, so a plain C2C is one HBM read and
+// one HBM write per element.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace bfly_dev {
+
+// ------------------------------------------------------------ complex ops
+__device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+
+// a * w for DIR = -1, a * conj(w) for DIR = +1 (w is a forward twiddle).
+template <int DIR>
+__device__ __forceinline__ float2 cmul_tw(float2 a, float2 w) {
+    if constexpr (DIR < 0)
+        return make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+    else
+        return make_float2(a.x * w.x + a.y * w.y, a.y * w.x - a.x * w.y);
+}
+
+// Multiply by e^{DIR * 2 pi i m / 16}.  m is a compile-time constant after
+// unrolling, so every branch folds away; the quarter turns cost no flops and
+// the eighth turns two multiplies.
+__device__ __forceinline__ float cos16(int m) {
+    switch (m & 15) {
+        case 0: return 1.0f;
+        case 1: case 15: return 0.92387953251128675613f;
+        case 2: case 14: return 0.70710678118654752440f;
+        case 3: case 13: return 0.38268343236508977173f;
+        case 4: case 12: return 0.0f;
+        case 5: case 11: return -0.38268343236508977173f;
+        case 6: case 10: return -0.70710678118654752440f;
+        case 7: case 9: return -0.92387953251128675613f;
+        default: return -1.0f;
+    }
+}
+__device__ __forceinline__ float sin16(int m) { return cos16(m - 4); }
+
+template <int DIR>
+__device__ __forceinline__ float2 rot16(float2 a, int m) {
+    m &= 15;
+    if (m == 0) return a;
+    if (m == 8) return make_float2(-a.x, -a.y);
+    // e^{DIR i pi/2} = DIR*i: (x + iy) * (DIR i) = -DIR y + i DIR x
+    if (m == 4) return DIR < 0 ? make_float2(a.y, -a.x) : make_float2(-a.y, a.x);
+    if (m == 12) return DIR < 0 ? make_float2(-a.y, a.x) : make_float2(a.y, -a.x);
+    const float c = cos16(m), s = DIR < 0 ? -sin16(m) : sin16(m);
+    if ((m & 3) == 2) {
+        // |c| == |s| == sqrt(1/2): two adds then two multiplies
+        const float h = c;  // +-sqrt(1/2)
+        const float g = s / c;  // +-1 exactly
+        return make_float2(h * (a.x - g * a.y), h * (g * a.x + a.y));
+    }
+    return make_float2(a.x * c - a.y * s, a.x * s + a.y * c);
+}
+
+// ---------------------------------------------------------- small DFTs
+// In-place DFT of R points held in registers, natural order in and out.
+template <int R, int DIR>
+__device__ __forceinline__ void dft(float2* v);
+
+template <>
+__device__ __forceinline__ void dft<1, -1>(float2*) {}
+template <>
+__device__ __forceinline__ void dft<1, 1>(float2*) {}
+
+template <int DIR>
+__device__ __forceinline__ void dft2(float2* v) {
+    const float2 a = v[0], b = v[1];
+    v[0] = cadd(a, b);
+    v[1] = csub(a, b);
+}
+template <>
+__device__ __forceinline__ void dft<2, -1>(float2* v) { dft2<-1>(v); }
+template <>
+__device__ __forceinline__ void dft<2, 1>(float2* v) { dft2<1>(v); }
+
+template <int DIR>
+__device__ __forceinline__ void dft4(float2* v) {
+    const float2 t0 = cadd(v[0], v[2]), t1 = csub(v[0], v[2]);
+    const float2 t2 = cadd(v[1], v[3]);
+    const float2 d = csub(v[1], v[3]);
+    const float2 t3 = DIR < 0 ? make_float2(d.y, -d.x) : make_float2(-d.y, d.x);
+    v[0] = cadd(t0, t2);
+    v[2] = csub(t0, t2);
+    v[1] = cadd(t1, t3);
+    v[3] = csub(t1, t3);
+}
+template <>
+__device__ __forceinline__ void dft<4, -1>(float2* v) { dft4<-1>(v); }
+template <>
+__device__ __forceinline__ void dft<4, 1>(float2* v) { dft4<1>(v); }
+
+// R = A * B with A = 4: n = B n1 + n2, k = k1 + A k2.
+//   Y[k1][n2] = W_R^{n2 k1} * DFT_A over n1 of x[B n1 + n2]
+//   X[k1 + A k2] = DFT_B over n2 of Y[k1][n2]
+template <int R, int DIR>
+__device__ __forceinline__ void dft_split(float2* v) {
+    constexpr int A = 4, B = R / 4;
+    float2 t[R];
+#pragma unroll
+    for (int n2 = 0; n2 < B; n2++) {
+        float2 c[A];
+#pragma unroll
+        for (int n1 = 0; n1 < A; n1++) c[n1] = v[B * n1 + n2];
+        dft4<DIR>(c);
+#pragma unroll
+        for (int k1 = 0; k1 < A; k1++) t[B * k1 + n2] = rot16<DIR>(c[k1], (n2 * k1) * (16 / R));
+    }
+#pragma unroll
+    for (int k1 = 0; k1 < A; k1++) {
+        float2 c[B];
+#pragma unroll
+        for (int n2 = 0; n2 < B; n2++) c[n2] = t[B * k1 + n2];
+        dft<B, DIR>(c);
+#pragma unroll
+        for (int k2 = 0; k2 < B; k2++) v[k1 + A * k2] = c[k2];
+    }
+}
+template <>
+__device__ __forceinline__ void dft<8, -1>(float2* v) { dft_split<8, -1>(v); }
+template <>
+__device__ __forceinline__ void dft<8, 1>(float2* v) { dft_split<8, 1>(v); }
+template <>
+__device__ __forceinline__ void dft<16, -1>(float2* v) { dft_split<16, -1>(v); }
+template <>
+__device__ __forceinline__ void dft<16, 1>(float2* v) { dft_split<16, 1>(v); }
+
+// ------------------------------------------------------------ LDS layout
+// One pad slot every 2^PAD_SHIFT complex values breaks the power-of-two
+// strides of the Stockham writes across LDS banks.
+constexpr int PAD_SHIFT = 4;
+__device__ __forceinline__ int lds_idx(int i) { return i + (i >> PAD_SHIFT); }
+__host__ __device__ constexpr int lds_len(int n) { return n + (n >> PAD_SHIFT); }
+
+// ------------------------------------------------------- Stockham passes
+// Radix plans: the list of radices of one length, first pass first.
+template <int... R>
+struct Radices {};
+
+template <int... R>
+struct RadixProduct;
+template <>
+struct RadixProduct<> { static constexpr int value = 1; };
+template <int R0, int... R>
+struct RadixProduct<R0, R...> { static constexpr int value = R0 * RadixProduct<R...>::value; };
+
+// Twiddle-table length (complex) of the passes after the first: pass s reads
+// W_{Ns R_s}^{r k} at [k * R_s + r] for k < Ns, r < R_s.
+template <int Ns, int... R>
+struct TwiddleLen;
+template <int Ns>
+struct TwiddleLen<Ns> { static constexpr int value = 0; };
+template <int Ns, int R0, int... R>
+struct TwiddleLen<Ns, R0, R...> {
+    static constexpr int value = (Ns > 1 ? Ns * R0 : 0) + TwiddleLen<Ns * R0, R...>::value;
+};
+
+template <int R0, int... R>
+struct First { static constexpr int value = R0; };
+
+// Runs the passes of radix list <R, Rest...> with Ns points already combined.
+// Thread t of a transform owns butterflies j = t + m*T (m < P/R) of each
+// pass.  On entry v[m*R + r] = x[t + m*T + r*N/R] (the first pass's operands);
+// on exit v[m*RL + r] = X[t + m*T + r*N/RL] for the last radix RL.
+// `lds` is this transform's region (lds_len(N) complex); `tw` the table of
+// the current pass (unused for the first).
+template <int N, int P, int T, int DIR, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stockham(float2 (&v)[P], float2* lds, const float2* __restrict__ tw,
+                                         int t) {
+    static_assert(P % R == 0, "radix must divide the points per thread");
+    constexpr int M = P / R;
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int k = (t + m * T) & (Ns - 1);
+            const float4* w4 = reinterpret_cast<const float4*>(tw + k * R);
+#pragma unroll
+            for (int q = 0; q < R / 2; q++) {
+                const float4 w = w4[q];
+                if (q > 0) v[m * R + 2 * q] = cmul_tw<DIR>(v[m * R + 2 * q], make_float2(w.x, w.y));
+                v[m * R + 2 * q + 1] = cmul_tw<DIR>(v[m * R + 2 * q + 1], make_float2(w.z, w.w));
+            }
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) dft<R, DIR>(&v[m * R]);
+    if constexpr (sizeof...(Rest) > 0) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int j = t + m * T;
+            const int k = j & (Ns - 1);
+            const int base = (j - k) * R + k;
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lds_idx(base + r * Ns)] = v[m * R + r];
+        }
+        __syncthreads();
+        constexpr int R2 = First<Rest...>::value;
+        constexpr int M2 = P / R2;
+#pragma unroll
+        for (int m = 0; m < M2; m++)
+#pragma unroll
+            for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lds_idx(t + m * T + r * (N / R2))];
+        __syncthreads();
+        stockham<N, P, T, DIR, Ns * R, Rest...>(v, lds, tw + (Ns > 1 ? Ns * R : 0), t);
+    }
+}
+
+}  // namespace bfly_dev

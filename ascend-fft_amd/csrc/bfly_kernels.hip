@@ -1,0 +1,402 @@
+// bfly_kernels.hip -- gfx950 FFT kernels and their launchers.
+//
+//   k_c2c_local  C2C, n <= 4096: one HBM read + one HBM write per element;
+//                256/T transforms per 256-thread workgroup (T = n/16).
+//   k_r2c        R2C as an n/2-point complex FFT with the split post-twiddle
+//                fused behind it; writes the n/2+1 bins at row stride n+2.
+//   k_c2r        C2R: Hermitian pre-twiddle fused in front of an n/2-point
+//                inverse complex FFT; writes reals scaled 1/n.
+//   k_long_cols  four-step, kernel 1: N1-point column FFTs x W_N^{n2 k1},
+//                written row-major by k1 into the workspace.
+//   k_long_rows  four-step, kernel 2: N2-point row FFTs, natural-order
+//                stores X[k1 + N1 k2].
+//   k_copy       streaming float4 copy (HBM ceiling probe).
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "bfly_device.hpp"
+#include "bfly_kernels.hpp"
+#include "bfly_plans.hpp"
+#include "bfly_tables.hpp"
+
+using namespace bfly_dev;
+using bfly_plan::kPlans;
+
+namespace {
+
+template <int I>
+struct PlanOf {
+    static constexpr int N = kPlans[I].n;
+    static constexpr int NRAD = kPlans[I].nrad;
+    static constexpr int R0 = kPlans[I].rad[0];
+    static constexpr int RL = kPlans[I].rad[NRAD - 1];
+    static constexpr int P = bfly_plan::kPoints;
+    static constexpr int T = N / P;  // threads per transform
+};
+
+template <int I, int T, int DIR, size_t... S>
+__device__ __forceinline__ void run_plan(float2 (&v)[bfly_plan::kPoints], float2* lds,
+                                         const float2* __restrict__ tw, int t,
+                                         std::index_sequence<S...>) {
+    stockham<kPlans[I].n, bfly_plan::kPoints, T, DIR, 1, kPlans[I].rad[S]...>(v, lds, tw, t);
+}
+
+template <int I, int T, int DIR>
+__device__ __forceinline__ void fft_local(float2 (&v)[bfly_plan::kPoints], float2* lds,
+                                          const float2* __restrict__ tw, int t) {
+    run_plan<I, T, DIR>(v, lds, tw, t, std::make_index_sequence<kPlans[I].nrad>{});
+}
+
+// ------------------------------------------------------------------ C2C
+template <int I, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+    uint32_t batch) {
+    using PL = PlanOf<I>;
+    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    __shared__ float2 lds[B * lds_len(N)];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const bool active = tr < batch;
+    float2 v[P];
+    if (active) {
+        const float2* src = in + tr * N;
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = src[t + m * T + r * (N / R0)];
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
+    }
+    fft_local<I, T, DIR>(v, lds + bl * lds_len(N), tw, t);
+    if (active) {
+        float2* dst = out + tr * N;
+#pragma unroll
+        for (int m = 0; m < P / RL; m++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) dst[t + m * T + r * (N / RL)] = v[m * RL + r];
+    }
+}
+
+// ------------------------------------------------------------------ R2C
+// z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
+//   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
+//   X[k] = E + W_n^k O            (Z[M] == Z[0]).
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restrict__ in,
+                                                           float* __restrict__ out,
+                                                           const float2* __restrict__ tw,
+                                                           const float2* __restrict__ wn,
+                                                           uint32_t batch) {
+    using PL = PlanOf<I>;
+    constexpr int M = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    __shared__ float2 lds[B * lds_len(M)];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const bool active = tr < batch;
+    float2* L = lds + bl * lds_len(M);
+    float2 v[P];
+    if (active) {
+        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)(2 * M));
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = src[t + m * T + r * (M / R0)];
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
+    }
+    fft_local<I, T, -1>(v, L, tw, t);
+    // Every LDS read of the last exchange completed before the barrier that
+    // ends it, so the spectrum can be parked immediately.
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
+    __syncthreads();
+    if (!active) return;
+    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M + 2));
+#pragma unroll
+    for (int i = 0; i <= P; i++) {
+        const int k = t + i * T;
+        if (k <= M) {
+            const float2 a = L[lds_idx(k & (M - 1))];
+            const float2 b = L[lds_idx((M - k) & (M - 1))];
+            const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+            // O = (a - conj b) / (2i) = (0.5 (a.y + b.y), -0.5 (a.x - b.x))
+            const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+            const float2 w = wn[k];
+            const float2 wo = cmul_tw<-1>(o, w);
+            dst[k] = cadd(e, wo);
+        }
+    }
+}
+
+// ------------------------------------------------------------------ C2R
+// Z'[k] = (X[k] + conj X[M-k]) + i W_n^{-k} (X[k] - conj X[M-k]), k < M,
+// with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restrict__ in,
+                                                           float* __restrict__ out,
+                                                           const float2* __restrict__ tw,
+                                                           const float2* __restrict__ wn,
+                                                           uint32_t batch, float scale) {
+    using PL = PlanOf<I>;
+    constexpr int M = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
+    __shared__ float2 lds[B * lds_len(M)];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const bool active = tr < batch;
+    float2* L = lds + bl * lds_len(M);
+    if (active) {
+        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)(2 * M + 2));
+#pragma unroll
+        for (int i = 0; i <= P; i++) {
+            const int k = t + i * T;
+            if (k <= M) L[k] = src[k];
+        }
+    }
+    __syncthreads();
+    float2 v[P];
+#pragma unroll
+    for (int m = 0; m < P / R0; m++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int k = t + m * T + r * (M / R0);
+            float2 a = L[k], b = L[M - k];
+            if (k == 0) { a.y = 0.f; b.y = 0.f; }
+            const float2 s = make_float2(a.x + b.x, a.y - b.y);
+            const float2 d = make_float2(a.x - b.x, a.y + b.y);
+            const float2 wd = cmul_tw<1>(d, wn[k]);  // W_n^{-k} d
+            v[m * R0 + r] = make_float2(s.x - wd.y, s.y + wd.x);
+        }
+    __syncthreads();
+    fft_local<I, T, 1>(v, L, tw, t);
+    if (!active) return;
+    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M));
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const float2 z = v[m * RL + r];
+            dst[t + m * T + r * (M / RL)] = make_float2(z.x * scale, z.y * scale);
+        }
+}
+
+// ------------------------------------------------------------- long C2C
+// n = N2 n1 + n2, k = k1 + N1 k2:
+//   Y[k1][n2] = W_N^{n2 k1} * FFT_N1 over n1 of x[N2 n1 + n2]   (k_long_cols)
+//   X[k1 + N1 k2] = FFT_N2 over n2 of Y[k1][n2]                 (k_long_rows)
+// Both kernels put the C columns (rows) of one workgroup on consecutive
+// lanes, so every global access is a run of C consecutive complex values.
+// Per-column LDS regions are spaced lds_len + 1 apart to spread banks.
+template <int I1, int N2, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
+    const float2* __restrict__ in, float2* __restrict__ ws, const float2* __restrict__ tw1,
+    const float2* __restrict__ wN) {
+    using PL = PlanOf<I1>;
+    constexpr int N1 = PL::N, T = PL::T, C = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int STRIDE = lds_len(N1) + 1;
+    constexpr int GROUPS = N2 / C;
+    static_assert(N2 % C == 0, "column groups must tile N2");
+    __shared__ float2 lds[C * STRIDE];
+    const int tid = threadIdx.x;
+    const int c = tid % C, t = tid / C;
+    const uint64_t b = blockIdx.x / GROUPS;
+    const int n2 = (blockIdx.x % GROUPS) * C + c;
+    constexpr uint64_t N = (uint64_t)N1 * N2;
+    const float2* src = in + b * N + n2;
+    float2 v[P];
+#pragma unroll
+    for (int m = 0; m < P / R0; m++)
+#pragma unroll
+        for (int r = 0; r < R0; r++)
+            v[m * R0 + r] = src[(uint64_t)N2 * (t + m * T + r * (N1 / R0))];
+    fft_local<I1, T, DIR>(v, lds + c * STRIDE, tw1, t);
+    float2* dst = ws + b * N + n2;
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int k1 = t + m * T + r * (N1 / RL);
+            dst[(uint64_t)N2 * k1] = cmul_tw<DIR>(v[m * RL + r], wN[n2 * k1]);
+        }
+}
+
+template <int I2, int N1, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_long_rows(
+    const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2) {
+    using PL = PlanOf<I2>;
+    constexpr int N2 = PL::N, T = PL::T, C = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int STRIDE = lds_len(N2) + 1;
+    constexpr int GROUPS = N1 / C;
+    static_assert(N1 % C == 0, "row groups must tile N1");
+    __shared__ float2 lds[C * STRIDE];
+    const int tid = threadIdx.x;
+    const uint64_t b = blockIdx.x / GROUPS;
+    const int k1_0 = (blockIdx.x % GROUPS) * C;
+    constexpr uint64_t N = (uint64_t)N1 * N2;
+    // Stage the C contiguous rows through LDS (row-linear, coalesced).
+    const float2* src = ws + b * N + (uint64_t)k1_0 * N2;
+#pragma unroll
+    for (int i = 0; i < C * N2 / bfly_plan::kBlock; i++) {
+        const int idx = tid + i * bfly_plan::kBlock;
+        const int row = idx / N2, col = idx % N2;
+        lds[row * STRIDE + lds_idx(col)] = src[idx];
+    }
+    __syncthreads();
+    const int c = tid % C, t = tid / C;
+    float2* L = lds + c * STRIDE;
+    float2 v[P];
+#pragma unroll
+    for (int m = 0; m < P / R0; m++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) v[m * R0 + r] = L[lds_idx(t + m * T + r * (N2 / R0))];
+    __syncthreads();
+    fft_local<I2, T, DIR>(v, L, tw2, t);
+    float2* dst = out + b * N + k1_0 + c;
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int k2 = t + m * T + r * (N2 / RL);
+            dst[(uint64_t)N1 * k2] = v[m * RL + r];
+        }
+}
+
+// ----------------------------------------------------------------- copy
+__global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
+                                              float4* __restrict__ dst, size_t n4) {
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n4; i += stride)
+        dst[i] = src[i];
+}
+
+// ------------------------------------------------------------ dispatch
+template <int I>
+hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uint32_t batch,
+                            int sign, hipStream_t s) {
+    constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
+    const dim3 grid((unsigned)((batch + B - 1) / B));
+    if (sign < 0)
+        hipLaunchKernelGGL((k_c2c_local<I, -1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
+    else
+        hipLaunchKernelGGL((k_c2c_local<I, 1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
+    return hipGetLastError();
+}
+
+template <int I>
+hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, const float2* wn,
+                       uint32_t batch, hipStream_t s) {
+    constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
+    const dim3 grid((unsigned)((batch + B - 1) / B));
+    if (kind == 1)
+        hipLaunchKernelGGL((k_r2c<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch);
+    else
+        hipLaunchKernelGGL((k_c2r<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch,
+                           1.0f / (float)(2 * PlanOf<I>::N));
+    return hipGetLastError();
+}
+
+template <int I1, int I2, int DIR>
+hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const float2* tw1,
+                           const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s) {
+    constexpr int N1 = kPlans[I1].n, N2 = kPlans[I2].n;
+    constexpr int C1 = bfly_plan::kBlock / PlanOf<I1>::T, C2 = bfly_plan::kBlock / PlanOf<I2>::T;
+    hipLaunchKernelGGL((k_long_cols<I1, N2, DIR>), dim3(batch * (N2 / C1)), dim3(bfly_plan::kBlock),
+                       0, s, in, ws, tw1, wN);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),
+                       0, s, ws, out, tw2);
+    return hipGetLastError();
+}
+
+template <int I1, int I2>
+hipError_t launch_long(const float2* in, float2* out, float2* ws, const float2* tw1,
+                       const float2* tw2, const float2* wN, uint32_t batch, int sign,
+                       hipStream_t s) {
+    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s)
+                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s);
+}
+
+constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
+
+}  // namespace
+
+namespace bfly_kern {
+
+hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+               const float* table, void* work, hipStream_t s) {
+    const float2* i2 = reinterpret_cast<const float2*>(in);
+    float2* o2 = reinterpret_cast<float2*>(out);
+    const float2* tw = reinterpret_cast<const float2*>(table);
+    switch (n) {
+        case 16: return launch_c2c_local<idx(16)>(i2, o2, tw, batch, sign, s);
+        case 32: return launch_c2c_local<idx(32)>(i2, o2, tw, batch, sign, s);
+        case 64: return launch_c2c_local<idx(64)>(i2, o2, tw, batch, sign, s);
+        case 128: return launch_c2c_local<idx(128)>(i2, o2, tw, batch, sign, s);
+        case 256: return launch_c2c_local<idx(256)>(i2, o2, tw, batch, sign, s);
+        case 512: return launch_c2c_local<idx(512)>(i2, o2, tw, batch, sign, s);
+        case 1024: return launch_c2c_local<idx(1024)>(i2, o2, tw, batch, sign, s);
+        case 2048: return launch_c2c_local<idx(2048)>(i2, o2, tw, batch, sign, s);
+        case 4096: return launch_c2c_local<idx(4096)>(i2, o2, tw, batch, sign, s);
+        default: break;
+    }
+    const bfly_host::TableLayout L = bfly_host::table_layout(0, n);
+    const float2 *t1 = tw + L.stage0, *t2 = tw + L.stage1, *wN = tw + L.extra;
+    float2* ws = reinterpret_cast<float2*>(work);
+    switch (n) {
+        case 8192: return launch_long<idx(64), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
+        case 16384: return launch_long<idx(128), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
+        case 32768: return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
+        case 65536: return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
+                const float* table, hipStream_t s) {
+    const float2* tw = reinterpret_cast<const float2*>(table);
+    const bfly_host::TableLayout L = bfly_host::table_layout(kind, n);
+    const float2* wn = tw + L.extra;
+    switch (n / 2) {
+        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, s);
+        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, s);
+        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, s);
+        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, s);
+        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, s);
+        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, s);
+        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, s);
+        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, s);
+        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
+    const size_t n4 = bytes / 16;
+    int dev = 0, cus = 256;
+    hipGetDevice(&dev);
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    size_t blocks = (n4 + 255) / 256;
+    const size_t cap = (size_t)cus * 16;
+    if (blocks > cap) blocks = cap;
+    if (blocks == 0) blocks = 1;
+    hipLaunchKernelGGL(k_copy, dim3((unsigned)blocks), dim3(256), 0, s,
+                       reinterpret_cast<const float4*>(src), reinterpret_cast<float4*>(dst), n4);
+    return hipGetLastError();
+}
+
+}  // namespace bfly_kern

@@ -1,0 +1,15 @@
+// bfly_kernels.hpp -- internal launchers of bfly_kernels.hip (not exported).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bfly_kern {
+
+hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+               const float* table, void* work, hipStream_t s);
+hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
+                const float* table, hipStream_t s);
+hipError_t copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+
+}  // namespace bfly_kern

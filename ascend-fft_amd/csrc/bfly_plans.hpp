@@ -1,0 +1,73 @@
+// bfly_plans.hpp -- the radix plans the gfx950 kernels are instantiated for,
+// shared by the device code (template arguments) and the host (twiddle-table
+// layout, launch geometry), so the two can never disagree.
+//
+// A plan for length N lists its radices first pass first.  Every thread owns
+// P = 16 points, so a transform uses T = N/16 threads and a 256-thread
+// workgroup holds 256/T transforms (one for N = 4096).  Radix 16 everywhere it
+// divides, the remainder as the last pass: 4096 = 16*16*16 is three register
+// passes with two LDS exchanges.
+#pragma once
+#include <stddef.h>
+#include <stdint.h>
+
+namespace bfly_plan {
+
+constexpr int kPoints = 16;       // points per thread
+constexpr int kBlock = 256;       // threads per workgroup for N <= 4096
+constexpr int kMaxRadices = 5;
+constexpr int kMinN = 16;
+constexpr int kMaxLocalN = 4096;  // largest length held by one 256-thread workgroup
+
+struct Plan {
+    int n;
+    int nrad;
+    int rad[kMaxRadices];
+};
+
+constexpr Plan kPlans[] = {
+    {16, 1, {16, 0, 0, 0, 0}},
+    {32, 2, {16, 2, 0, 0, 0}},
+    {64, 2, {16, 4, 0, 0, 0}},
+    {128, 2, {16, 8, 0, 0, 0}},
+    {256, 2, {16, 16, 0, 0, 0}},
+    {512, 3, {16, 16, 2, 0, 0}},
+    {1024, 3, {16, 16, 4, 0, 0}},
+    {2048, 3, {16, 16, 8, 0, 0}},
+    {4096, 3, {16, 16, 16, 0, 0}},
+};
+constexpr int kNumPlans = (int)(sizeof(kPlans) / sizeof(kPlans[0]));
+
+constexpr int plan_index(uint32_t n) {
+    for (int i = 0; i < kNumPlans; i++)
+        if ((uint32_t)kPlans[i].n == n) return i;
+    return -1;
+}
+
+// Complex entries of the inter-pass twiddle table of plan i:
+// sum over passes s >= 1 of Ns * R_s, Ns = R_0 * ... * R_{s-1}.
+constexpr size_t stage_twiddle_len(int i) {
+    size_t len = 0, ns = 1;
+    for (int s = 0; s < kPlans[i].nrad; s++) {
+        if (s > 0) len += ns * (size_t)kPlans[i].rad[s];
+        ns *= (size_t)kPlans[i].rad[s];
+    }
+    return len;
+}
+
+constexpr int threads_per_transform(uint32_t n) { return (int)(n / kPoints); }
+
+// Four-step split of a long length into two local lengths N1 * N2 (column
+// FFTs of N1 first, then row FFTs of N2).  Balanced, N2 >= N1.
+constexpr bool long_split(uint32_t n, uint32_t* n1, uint32_t* n2) {
+    if (n <= (uint32_t)kMaxLocalN || n > (uint32_t)kMaxLocalN * (uint32_t)kMaxLocalN) return false;
+    uint32_t l = 0;
+    while ((1u << l) < n) l++;
+    if ((1u << l) != n) return false;
+    const uint32_t a = 1u << (l / 2), b = n / a;
+    if (n1) *n1 = a;
+    if (n2) *n2 = b;
+    return a >= 16 && b <= (uint32_t)kMaxLocalN;
+}
+
+}  // namespace bfly_plan

@@ -1,0 +1,129 @@
+// bfly_stub.cpp -- CPU stub of the C ABI (include/bfly_hip.h) for host-only
+// tests: the runtime calls are malloc/memcpy with counters and failure
+// injection, the execute calls run the real argument guards and count a launch
+// without computing anything.  Same role as the reference's fake AscendCL
+// (tests/stubs/acl/acl_rt.h:15-53 of the reference): it lets the C++
+// Context/Plan guard and cache-invalidation tests run without a GPU.
+#include <cstdint>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+
+#include "bfly_hip.h"
+#include "bfly_tables.hpp"
+
+namespace {
+struct Counters {
+    int allocations = 0, frees = 0, copies = 0, launches = 0;
+    int streams_created = 0, streams_destroyed = 0, inits = 0, syncs = 0;
+    int fail_allocation = 0;  // the Nth allocation fails (0 = never)
+    int fail_copy = 0;        // the Nth copy fails (0 = never)
+    int fail_launch = 0;      // the Nth launch fails (0 = never)
+} g;
+int launch() {
+    ++g.launches;
+    return g.launches == g.fail_launch ? BFLY_ERR_LAUNCH : BFLY_OK;
+}
+}  // namespace
+
+extern "C" {
+
+// ----------------------------------------------------- stub-only controls
+BFLY_API void bfly_stub_reset(void) { g = Counters{}; }
+BFLY_API void bfly_stub_set_failures(int allocation, int copy, int launch_) {
+    g.fail_allocation = allocation;
+    g.fail_copy = copy;
+    g.fail_launch = launch_;
+}
+// out[0..7] = allocations, frees, copies, launches, streams_created,
+// streams_destroyed, inits, syncs
+BFLY_API void bfly_stub_counters(int* out) {
+    const int v[8] = {g.allocations, g.frees, g.copies, g.launches,
+                      g.streams_created, g.streams_destroyed, g.inits, g.syncs};
+    std::memcpy(out, v, sizeof(v));
+}
+
+// ------------------------------------------------------------- runtime
+int bfly_rt_init(int device) {
+    ++g.inits;
+    return device == 0 ? BFLY_OK : BFLY_ERR_TABLE;
+}
+int bfly_rt_stream_create(void** stream) {
+    if (!stream) return BFLY_ERR_SHAPE;
+    ++g.streams_created;
+    *stream = reinterpret_cast<void*>(static_cast<uintptr_t>(0x1000 + g.streams_created));
+    return BFLY_OK;
+}
+int bfly_rt_stream_destroy(void* stream) {
+    if (stream) ++g.streams_destroyed;
+    return BFLY_OK;
+}
+int bfly_rt_stream_sync(void*) {
+    ++g.syncs;
+    return BFLY_OK;
+}
+int bfly_rt_malloc(void** ptr, size_t bytes) {
+    if (!ptr) return BFLY_ERR_SHAPE;
+    ++g.allocations;
+    if (g.allocations == g.fail_allocation) { *ptr = nullptr; return BFLY_ERR_ALLOC; }
+    *ptr = std::malloc(bytes ? bytes : 16);
+    return *ptr ? BFLY_OK : BFLY_ERR_ALLOC;
+}
+int bfly_rt_free(void* ptr) {
+    if (ptr) { ++g.frees; std::free(ptr); }
+    return BFLY_OK;
+}
+int bfly_rt_memcpy_h2d(void* dst, const void* src, size_t bytes, void*) {
+    ++g.copies;
+    if (g.copies == g.fail_copy || !dst || !src) return BFLY_ERR_H2D;
+    std::memcpy(dst, src, bytes);
+    return BFLY_OK;
+}
+int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void*) {
+    ++g.copies;
+    if (g.copies == g.fail_copy || !dst || !src) return BFLY_ERR_D2H;
+    std::memcpy(dst, src, bytes);
+    return BFLY_OK;
+}
+
+// ------------------------------------------------------------- execute
+int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+             const float* table, void* work, void*) {
+    if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return e;
+    return launch();
+}
+int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
+    if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;
+    return launch();
+}
+int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
+    if (int e = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return e;
+    return launch();
+}
+int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,
+                           uint32_t batch, const float* table, void* work, int reps, void* stream,
+                           float* ms) {
+    if (!ms || reps <= 0) return BFLY_ERR_SHAPE;
+    for (int i = 0; i < reps; i++) {
+        int rc = kind == BFLY_KIND_C2C ? bfly_c2c(in, out, n, batch, sign, table, work, stream)
+                 : kind == BFLY_KIND_R2C ? bfly_r2c(in, out, n, batch, table, stream)
+                 : kind == BFLY_KIND_C2R ? bfly_c2r(in, out, n, batch, table, stream)
+                                         : BFLY_ERR_SHAPE;
+        if (rc) return rc;
+    }
+    *ms = 0.001f * (float)reps;
+    return BFLY_OK;
+}
+
+// --------------------------------------------------------------- probes
+int bfly_probe_copy(const void* src, void* dst, size_t bytes, void*) {
+    if (!src || !dst || bytes == 0 || (bytes & 15)) return BFLY_ERR_SHAPE;
+    return launch();
+}
+int bfly_probe_device(int device, char* json, size_t cap) {
+    if (!json || cap == 0 || device != 0) return BFLY_ERR_SHAPE;
+    const int w = std::snprintf(json, cap, "{\"name\": \"cpu-stub\", \"gcn_arch\": \"none\"}");
+    return (w > 0 && (size_t)w < cap) ? BFLY_OK : BFLY_ERR_SHAPE;
+}
+
+}  // extern "C"

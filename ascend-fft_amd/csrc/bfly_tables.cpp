@@ -1,0 +1,154 @@
+// bfly_tables.cpp -- host-side shape rules, table layout and table generation
+// of the C ABI (include/bfly_hip.h).  Pure host C++: linked into
+// libbfly_hip.so and into the CPU stub library, so both agree byte for byte.
+//
+// Twiddles are evaluated in FP64 from the exact rational angle 2*pi*m/L
+// (integer m reduced modulo L first) and rounded once to fp32.
+#include <cmath>
+#include <cstddef>
+#include <cstdint>
+
+#include "bfly_hip.h"
+#include "bfly_plans.hpp"
+#include "bfly_tables.hpp"
+
+namespace bfly_host {
+
+static inline bool pow2(uint32_t n) { return n && !(n & (n - 1)); }
+
+bool shape_ok(int kind, uint32_t n) {
+    if (!pow2(n)) return false;
+    switch (kind) {
+        case BFLY_KIND_C2C:
+            if (n <= (uint32_t)bfly_plan::kMaxLocalN) return bfly_plan::plan_index(n) >= 0;
+            return bfly_plan::long_split(n, nullptr, nullptr) && n <= 65536u;
+        case BFLY_KIND_R2C:
+        case BFLY_KIND_C2R:
+            return n >= 32u && n <= 2u * (uint32_t)bfly_plan::kMaxLocalN;
+        default:
+            return false;
+    }
+}
+
+// FP64 e^{-2 pi i m / L} rounded to fp32.
+static inline void twiddle(uint64_t m, uint64_t L, float* out) {
+    m %= L;
+    const double ang = -2.0 * M_PI * (double)m / (double)L;
+    out[0] = (float)std::cos(ang);
+    out[1] = (float)std::sin(ang);
+}
+
+// Inter-pass table of local plan i, complex entries at [k * R_s + r].
+static size_t fill_stage(int i, float* dst) {
+    const auto& p = bfly_plan::kPlans[i];
+    size_t off = 0;
+    uint64_t ns = 1;
+    for (int s = 0; s < p.nrad; s++) {
+        const uint64_t R = (uint64_t)p.rad[s];
+        if (s > 0) {
+            for (uint64_t k = 0; k < ns; k++)
+                for (uint64_t r = 0; r < R; r++) {
+                    if (dst) twiddle(r * k, ns * R, dst + 2 * (off + k * R + r));
+                }
+            off += ns * R;
+        }
+        ns *= R;
+    }
+    return off;
+}
+
+TableLayout table_layout(int kind, uint32_t n) {
+    TableLayout t{};
+    if (!shape_ok(kind, n)) return t;
+    if (kind == BFLY_KIND_C2C && n <= (uint32_t)bfly_plan::kMaxLocalN) {
+        t.stage0 = 0;
+        t.complex_len = bfly_plan::stage_twiddle_len(bfly_plan::plan_index(n));
+        if (t.complex_len == 0) t.complex_len = 2;  // single-pass plan: no twiddles
+    } else if (kind == BFLY_KIND_C2C) {
+        uint32_t n1 = 0, n2 = 0;
+        bfly_plan::long_split(n, &n1, &n2);
+        t.stage0 = 0;
+        t.stage1 = bfly_plan::stage_twiddle_len(bfly_plan::plan_index(n1));
+        t.extra = t.stage1 + bfly_plan::stage_twiddle_len(bfly_plan::plan_index(n2));
+        t.complex_len = t.extra + n;  // W_N^m, m < N
+    } else {
+        const uint32_t m = n / 2;
+        t.stage0 = 0;
+        t.extra = bfly_plan::stage_twiddle_len(bfly_plan::plan_index(m));
+        t.complex_len = t.extra + m + 2;  // W_n^k, k = 0..m, padded to even
+    }
+    return t;
+}
+
+int fill_table(int kind, uint32_t n, float* dst) {
+    if (!dst || !shape_ok(kind, n)) return BFLY_ERR_SHAPE;
+    const TableLayout t = table_layout(kind, n);
+    for (size_t i = 0; i < 2 * t.complex_len; i++) dst[i] = 0.f;
+    if (kind == BFLY_KIND_C2C && n <= (uint32_t)bfly_plan::kMaxLocalN) {
+        fill_stage(bfly_plan::plan_index(n), dst);
+    } else if (kind == BFLY_KIND_C2C) {
+        uint32_t n1 = 0, n2 = 0;
+        bfly_plan::long_split(n, &n1, &n2);
+        fill_stage(bfly_plan::plan_index(n1), dst + 2 * t.stage0);
+        fill_stage(bfly_plan::plan_index(n2), dst + 2 * t.stage1);
+        for (uint64_t m = 0; m < n; m++) twiddle(m, n, dst + 2 * (t.extra + m));
+    } else {
+        const uint32_t m = n / 2;
+        fill_stage(bfly_plan::plan_index(m), dst);
+        for (uint64_t k = 0; k <= m; k++) twiddle(k, n, dst + 2 * (t.extra + k));
+    }
+    return BFLY_OK;
+}
+
+// Largest grid any launcher issues for (kind, n, batch) must fit a 32-bit x
+// dimension.
+static bool grid_ok(int kind, uint32_t n, uint32_t batch) {
+    uint64_t blocks = 0;
+    const uint64_t b = batch;
+    if (kind == BFLY_KIND_C2C && n > (uint32_t)bfly_plan::kMaxLocalN) {
+        uint32_t n1 = 0, n2 = 0;
+        bfly_plan::long_split(n, &n1, &n2);
+        const uint64_t c1 = bfly_plan::kBlock / bfly_plan::threads_per_transform(n1);
+        const uint64_t c2 = bfly_plan::kBlock / bfly_plan::threads_per_transform(n2);
+        const uint64_t g1 = b * (n2 / c1), g2 = b * (n1 / c2);
+        blocks = g1 > g2 ? g1 : g2;
+    } else {
+        const uint32_t local = kind == BFLY_KIND_C2C ? n : n / 2;
+        const uint64_t per = bfly_plan::kBlock / bfly_plan::threads_per_transform(local);
+        blocks = (b + per - 1) / per;
+    }
+    return blocks >= 1 && blocks <= 0x7fffffffull;
+}
+
+int check_execute(int kind, int sign, const void* in, const void* out, uint32_t n,
+                  uint32_t batch, const void* table, const void* work) {
+    if (!in || !out || batch == 0 || !shape_ok(kind, n)) return BFLY_ERR_SHAPE;
+    if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
+    if (!grid_ok(kind, n, batch)) return BFLY_ERR_SHAPE;
+    if (!table) return BFLY_ERR_TABLE;
+    if (kind == BFLY_KIND_C2C && n > (uint32_t)bfly_plan::kMaxLocalN && !work) return BFLY_ERR_SHAPE;
+    return BFLY_OK;
+}
+
+}  // namespace bfly_host
+
+extern "C" {
+
+int bfly_supported(int kind, uint32_t n) { return bfly_host::shape_ok(kind, n) ? 1 : 0; }
+
+size_t bfly_table_floats(int kind, uint32_t n) {
+    return 2 * bfly_host::table_layout(kind, n).complex_len;
+}
+
+int bfly_table_fill(int kind, uint32_t n, float* host_table) {
+    return bfly_host::fill_table(kind, n, host_table);
+}
+
+size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch) {
+    if (kind != BFLY_KIND_C2C || !bfly_host::shape_ok(kind, n) ||
+        n <= (uint32_t)bfly_plan::kMaxLocalN)
+        return 0;
+    return (size_t)n * (size_t)batch * 8u;
+}
+
+}  // extern "C"

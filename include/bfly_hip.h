@@ -1,0 +1,132 @@
+/*
+ * bfly_hip.h -- C ABI of libbfly_hip.so, the gfx950 (MI355X) FFT library.
+ *
+ * Plain C: pointers, sizes and int status codes only; no C++ or HIP types
+ * cross this boundary (a stream is an opaque void*, a hipStream_t underneath;
+ * NULL means the default stream).  The C++ bfly::Context / bfly::Plan API
+ * (include/butterfly/plan.hpp) is built on these functions and nothing else,
+ * and the same symbol set is implemented by a CPU stub for host-only tests.
+ *
+ * Data layouts are the reference's transform contract
+ * (docs/user-guide/transforms.md:15-44 of the reference):
+ *   C2C  interleaved complex  [batch][2n] floats  -> [batch][2n]
+ *   R2C  real                 [batch][n]          -> half spectrum [batch][n+2]
+ *   C2R  half spectrum        [batch][n+2]        -> real [batch][n], scaled 1/n
+ * Transforms are unnormalised except C2R.  Device pointers throughout the
+ * execute calls; in and out must not overlap.
+ *
+ * Status codes map 1:1 onto the reference's Plan::run* return codes
+ * (docs/reference/api.md:38-48, src/framework/butterfly.cpp:723-850).
+ */
+#ifndef BFLY_HIP_H
+#define BFLY_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#if defined(__GNUC__)
+#define BFLY_API __attribute__((visibility("default")))
+#else
+#define BFLY_API
+#endif
+
+#define BFLY_OK 0
+#define BFLY_ERR_NOT_LOADED (-1)  /* kernel for this transform not available   */
+#define BFLY_ERR_TABLE (-2)       /* table preparation failed                  */
+#define BFLY_ERR_ALLOC (-3)       /* device allocation failed                  */
+#define BFLY_ERR_H2D (-4)         /* host-to-device copy failed                */
+#define BFLY_ERR_LAUNCH (-5)      /* kernel launch failed                      */
+#define BFLY_ERR_SYNC (-6)        /* stream synchronisation failed             */
+#define BFLY_ERR_D2H (-7)         /* device-to-host copy failed                */
+#define BFLY_ERR_SHAPE (-8)       /* shape / pointer / size guard              */
+
+#define BFLY_KIND_C2C 0
+#define BFLY_KIND_R2C 1
+#define BFLY_KIND_C2R 2
+
+/* ------------------------------------------------------------- runtime
+ * Replaces the AscendCL runtime calls of the reference's Plan/Context:
+ * aclInit + aclrtSetDevice (src/framework/butterfly.cpp:574-575),
+ * aclrtCreateStream/DestroyStream (butterfly.cpp:576, 556),
+ * aclrtMalloc/aclrtFree (butterfly.cpp:257-260, 303-310),
+ * aclrtMemcpy H2D/D2H (butterfly.cpp:263-269, 355, 368),
+ * aclrtSynchronizeStream (butterfly.cpp:367). */
+BFLY_API int bfly_rt_init(int device);
+BFLY_API int bfly_rt_stream_create(void** stream);
+BFLY_API int bfly_rt_stream_destroy(void* stream);
+BFLY_API int bfly_rt_stream_sync(void* stream);
+BFLY_API int bfly_rt_malloc(void** ptr, size_t bytes);
+BFLY_API int bfly_rt_free(void* ptr);
+BFLY_API int bfly_rt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+BFLY_API int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+/* Elapsed milliseconds of `reps` back-to-back launches of one transform on
+ * `stream`, measured with HIP events (replaces the host wall-clock loop of
+ * Plan::measure, butterfly.cpp:852-924). */
+BFLY_API int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,
+                           uint32_t batch, const float* table, void* work, int reps,
+                           void* stream, float* ms);
+
+/* ---------------------------------------------------------- shape rules */
+/* 1 when the library has kernels for (kind, n), else 0.  C2C: n a power of
+ * two in [16, 65536]; R2C/C2R: n a power of two in [32, 8192]. */
+BFLY_API int bfly_supported(int kind, uint32_t n);
+
+/* ------------------------------------------------------- plan tables
+ * The input-independent tables one transform needs (FP64-computed twiddles
+ * stored as fp32), generated on the host and uploaded once per length.
+ * Replaces Generator::genTwiddles + genBitReverse* + genTransposePlane +
+ * genInterleave and genR2CTwiddles (butterfly.cpp:96-166, 692-703) and the
+ * long-FFT wT table (src/host/fft_check.cpp:524-536): the gfx950 kernels use
+ * Stockham autosort passes, so no permutation tables exist. */
+BFLY_API size_t bfly_table_floats(int kind, uint32_t n);
+BFLY_API int bfly_table_fill(int kind, uint32_t n, float* host_table);
+
+/* Device workspace (bytes) an execute call needs; 0 for every length that
+ * runs in one kernel.  Long C2C (n > 4096) needs n*batch complex values for
+ * the four-step intermediate. */
+BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
+
+/* --------------------------------------------------------------- execute
+ * Asynchronous on `stream`; device pointers.  `table` is the device copy of
+ * bfly_table_fill(kind, n); `work` is a device buffer of at least
+ * bfly_workspace_bytes(kind, n, batch) bytes (may be NULL when that is 0).
+ *
+ * bfly_c2c replaces the kfft_fwd launch of Plan::run (butterfly.cpp:723-749,
+ * kernel ABI butterfly.cpp:663-679) and, for n > 4096, the long-FFT launch
+ * chain of src/host/fft_check.cpp:846-970.  sign = -1 forward, +1
+ * unnormalised inverse (the reference's twiddle-sign flip + xflip,
+ * butterfly.cpp:625, src/ascendc/fft_radix2.cpp:64-69). */
+BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+             const float* table, void* work, void* stream);
+
+/* bfly_r2c replaces kfft_fwd(n/2) + kfft_r2c_post + the host repack of
+ * Plan::runR2C (butterfly.cpp:753-806); one fused kernel, output written
+ * directly at row stride n+2. */
+BFLY_API int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
+             void* stream);
+
+/* bfly_c2r replaces the host pad + kfft_c2r_prep + kfft_fwd(xflip) +
+ * kfft_c2r_post chain of Plan::runC2R (butterfly.cpp:810-850); one fused
+ * kernel.  The imaginary parts of the DC and Nyquist bins are ignored, as the
+ * reference's Hermitian expansion does (src/framework/reference.cpp:64-79). */
+BFLY_API int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
+             void* stream);
+
+/* ------------------------------------------------------------- probes */
+/* Device-to-device copy of `bytes` (16-byte aligned, multiple of 16) with
+ * the library's streaming copy kernel: the measured HBM ceiling next to the
+ * nominal peak (replaces src/host/bw_probe.cpp). */
+BFLY_API int bfly_probe_copy(const void* src, void* dst, size_t bytes, void* stream);
+
+/* Device facts for the gfx950 hardware profile (replaces
+ * src/host/profile.cpp).  Writes at most `cap` bytes of JSON into `json`. */
+BFLY_API int bfly_probe_device(int device, char* json, size_t cap);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BFLY_HIP_H */

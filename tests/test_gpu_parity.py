@@ -1,0 +1,224 @@
+"""GPU parity: libbfly_hip.so against the FP64 oracle on seeded inputs.
+
+Gates (per transform): relative L2 error <= 1e-5 against FP64, and the
+reference's maxRelScaled <= 1e-4 against its fp32-rounded FP64 result
+(include/butterfly/reference.hpp:30-41, used by src/host/fft_check.cpp:1186-1248).
+"""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs an MI355X")]
+
+REL_L2 = 1e-5
+MAX_REL_SCALED = 1e-4
+GRID_N = [64, 128, 256, 512, 1024, 2048, 4096]
+GRID_B = [1, 4, 16, 64, 256, 1024, 4096]
+
+
+def _ops():
+    from bfly import ops
+    return ops
+
+
+def run_c2c(x, n, sign=-1):
+    import torch
+    t = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    y = _ops().c2c(t, sign)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def gate_c2c(got, x, n, sign=-1):
+    ref = oracle.c2c_f64(x, n, sign)
+    e = oracle.rel_l2_rows(got, ref)
+    assert e <= REL_L2, f"n={n} rel-L2 {e:.3e}"
+    m = oracle.max_rel_scaled(got, ref.astype(np.float32))
+    assert m <= MAX_REL_SCALED, f"n={n} maxRelScaled {m:.3e}"
+    return e
+
+
+def test_cfg1_c2c_n1024_b64_seed0():
+    n, b = 1024, 64
+    x = oracle.uniform_input(0, (b, 2 * n))
+    got = run_c2c(x, n)
+    e = gate_c2c(got, x, n)
+    assert e < 1e-6  # an FP32 Stockham at n=1024 sits near 2e-7
+
+
+@pytest.mark.parametrize("n", GRID_N)
+def test_c2c_grid_seed1(n):
+    # One seeded draw per (n, batch) point of the 49-point grid.
+    for b in GRID_B:
+        x = oracle.uniform_input(1, (b, 2 * n))
+        got = run_c2c(x, n)
+        gate_c2c(got, x, n)
+
+
+@pytest.mark.parametrize("n", [16, 32, 64, 1024, 4096])
+def test_c2c_inverse_is_conjugate_transform(n):
+    b = 5
+    x = oracle.uniform_input(7, (b, 2 * n))
+    got = run_c2c(x, n, sign=+1)
+    gate_c2c(got, x, n, sign=+1)
+    back = run_c2c(run_c2c(x, n), n, sign=+1) / n
+    assert np.max(np.abs(back - x)) < 1e-5
+
+
+@pytest.mark.parametrize("n", [8192, 16384, 32768, 65536])
+def test_long_c2c_b64_seed2(n):
+    b = 64
+    x = oracle.uniform_input(2, (b, 2 * n))
+    got = run_c2c(x, n)
+    gate_c2c(got, x, n)
+
+
+def test_long_c2c_matches_host_four_step_split():
+    # The reference's host four-step chain (src/host/fft_check.cpp:904-963) in
+    # FP64 with its own split (fft_check.cpp:295-308) agrees with the direct
+    # FFT; the device result must agree with both.
+    n, b = 8192, 3
+    x = oracle.uniform_input(5, (b, 2 * n))
+    n1, n2 = oracle.long_split(n)
+    four = oracle.four_step_f64(x, n, n1, n2)
+    direct = oracle.c2c_f64(x, n)
+    assert np.max(np.abs(four - direct)) / np.max(np.abs(direct)) < 1e-12
+    got = run_c2c(x, n)
+    assert oracle.rel_l2_rows(got, four) <= REL_L2
+
+
+@pytest.mark.parametrize("n", [64, 1024, 4096, 65536])
+@pytest.mark.parametrize("mode", sorted(oracle.NUMERICAL_MODES))
+def test_numerical_modes(n, mode):
+    # src/host/fft_check.cpp:83-116 input modes, reference gate maxRelScaled.
+    b = 3
+    x = oracle.numerical_input(n, b, mode, seed=1)
+    got = run_c2c(x, n)
+    ref = oracle.c2c_f64(x, n).astype(np.float32)
+    assert oracle.max_rel_scaled(got, ref) <= MAX_REL_SCALED
+    if mode == "impulse":
+        for row in range(b):
+            k = np.arange(n)
+            want = np.exp(-2j * np.pi * k * (row % n) / n)
+            g = got[row, 0::2] + 1j * got[row, 1::2]
+            assert np.max(np.abs(g - want)) < 1e-5
+    if mode == "constant":
+        g = got[:, 0::2] + 1j * got[:, 1::2]
+        assert np.allclose(g[:, 0], n * (0.25 - 0.5j), rtol=1e-6)
+        assert np.max(np.abs(g[:, 1:])) < 1e-6 * n
+    if mode == "zero":
+        assert not np.any(got)
+
+
+@pytest.mark.parametrize("mode", ["ofdm", "radar", "dl"])
+def test_app_inputs(mode):
+    for n in (256, 4096):
+        x = oracle.app_input(n, 5, mode)
+        got = run_c2c(x, n)
+        ref = oracle.c2c_f64(x, n).astype(np.float32)
+        assert oracle.max_rel_scaled(got, ref) <= MAX_REL_SCALED
+
+
+@pytest.mark.parametrize("b", [1, 2, 3, 47, 48, 49, 95, 97, 143, 193])
+def test_ragged_batches_all_rows(b):
+    # tests/test_limits.cpp:249-261: odd batches around workgroup tails, all rows.
+    for n in (64, 1024):
+        x = oracle.pattern(b * 2 * n).reshape(b, 2 * n)
+        got = run_c2c(x, n)
+        gate_c2c(got, x, n)
+
+
+def test_pattern_input_reference_gate():
+    # The reference's own smoke (tests/test_framework.cpp:109-137): patternAt
+    # input, n=4096, batch=64, sampled rows against refFftF32.
+    n, b = 4096, 64
+    x = oracle.pattern(b * 2 * n).reshape(b, 2 * n)
+    got = run_c2c(x, n)
+    for row in (0, 9, 18, 27, 36, 45, 54, 63):
+        ref = oracle.ref_fft_f32(x[row], n)[0]
+        assert oracle.max_rel_scaled(got[row], ref) <= MAX_REL_SCALED
+
+
+# ----------------------------------------------------------------- R2C / C2R
+def run_r2c(x, n):
+    import torch
+    y = _ops().rfft(torch.from_numpy(np.ascontiguousarray(x)).cuda())
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+def run_c2r(h, n):
+    import torch
+    y = _ops().irfft(torch.from_numpy(np.ascontiguousarray(h)).cuda(), n)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [128, 256, 512, 1024, 2048, 4096, 8192])
+def test_r2c_grid_seed1(n):
+    for b in GRID_B:
+        x = oracle.uniform_input(1, (b, n))
+        got = run_r2c(x, n)
+        assert got.shape == (b, n + 2)
+        ref = oracle.r2c_f64(x, n)
+        e = oracle.rel_l2_rows(got, ref)
+        assert e <= REL_L2, f"r2c n={n} b={b} rel-L2 {e:.3e}"
+        assert oracle.max_rel_scaled(got, ref.astype(np.float32)) <= MAX_REL_SCALED
+
+
+@pytest.mark.parametrize("n", [64, 128, 256, 512, 1024, 2048, 4096])
+def test_c2r_grid_seed1(n):
+    # Half spectra built as the R2C of seeded reals (the grid's C2R inputs).
+    for b in GRID_B:
+        x = oracle.uniform_input(1, (b, n))
+        h = oracle.r2c_f64(x, n).astype(np.float32)
+        got = run_c2r(h, n)
+        assert got.shape == (b, n)
+        ref = oracle.c2r_f64(h, n)
+        e = oracle.rel_l2_rows(got, ref)
+        assert e <= REL_L2, f"c2r n={n} b={b} rel-L2 {e:.3e}"
+        assert np.max(np.abs(got - x)) < 1e-5  # round trip
+
+
+@pytest.mark.parametrize("n", [32, 64, 128, 4096, 8192])
+def test_c2r_arbitrary_half_spectrum(n):
+    # src/host/fft_check.cpp:179-192 half spectra: DC imaginary part nonzero,
+    # Nyquist imaginary 0; the reference's expansion ignores both imaginary
+    # parts (src/framework/reference.cpp:64-79).
+    b = 3
+    h = oracle.half_input(n, b)
+    got = run_c2r(h, n)
+    ref = oracle.c2r_f64(h, n)
+    assert oracle.rel_l2_rows(got, ref) <= REL_L2
+    for row in range(b):
+        r = np.empty(n, np.float32)
+        oracle.lib().orc_ref_c2r_f32(np.ascontiguousarray(h[row]), r, n)
+        assert oracle.max_rel_scaled(got[row], r) <= MAX_REL_SCALED
+
+
+@pytest.mark.parametrize("n", [64, 128, 4096, 8192])
+def test_real_boundary_round_trip(n):
+    # tests/test_framework.cpp:16-87: R2C/C2R at the envelope ends, B=3,
+    # patternAt input.
+    b = 3
+    x = oracle.pattern(b * n).reshape(b, n)
+    if n >= 128:
+        h = run_r2c(x, n)
+        for row in range(b):
+            r = np.empty(n + 2, np.float32)
+            oracle.lib().orc_ref_r2c_f32(np.ascontiguousarray(x[row]), r, n)
+            assert oracle.max_rel_scaled(h[row], r) <= MAX_REL_SCALED
+    else:
+        h = oracle.pattern(b * (n + 2)).reshape(b, n + 2).copy()
+        h[:, 1] = 0
+        h[:, n + 1] = 0
+    back = run_c2r(h, n)
+    for row in range(b):
+        r = np.empty(n, np.float32)
+        oracle.lib().orc_ref_c2r_f32(np.ascontiguousarray(h[row]), r, n)
+        assert oracle.max_rel_scaled(back[row], r) <= MAX_REL_SCALED
+        if n >= 128:
+            assert oracle.max_rel_scaled(back[row], x[row]) <= MAX_REL_SCALED
