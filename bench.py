@@ -1,0 +1,192 @@
+#!/usr/bin/env python3
+"""Headline benchmark: plain fp32 C2C forward, N=4096, batch=4096, on one MI355X.
+
+One step = one bfly_c2c launch over the whole batch ([4096][8192] float32,
+interleaved), inputs resident in HBM.  Prints ONE JSON line:
+  value          algorithmic GB/s = 16*N*batch bytes / mean step time
+  roofline       achieved GB/s of the kernel (HIP events on its stream) vs the
+                 8 TB/s HBM peak, plus HBM traffic per launch from the committed
+                 rocprofv3 counter summary (profiles/), or null
+  cpu_baseline   the reference's own FP64 CPU FFT (oracle/_ref, built from the
+                 reference sources) timed on this host on a bounded sample
+  chosen         the plan the measured selection picks for this shape, its
+                 speedup over the plain plan and its relative error vs FP64
+
+Timed steps rotate over several input/output buffer pairs (>= 1 GiB in
+total) so that the 256 MB last-level cache cannot serve one step's input from
+the previous step.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import re
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+sys.path.insert(0, str(ROOT / "ascend-fft_amd"))
+
+N = 4096
+BATCH = 4096
+HBM_PEAK_GBS = 8000.0
+METRIC = "C2C fp32 GB/s (%HBM peak) N=4096 b=4096; chosen/plain speedup @rel-err 1e-5"
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=200)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--rotate", type=int, default=4, help="input/output buffer pairs cycled")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--probe-copy", action="store_true",
+                    help="also run the streaming copy kernel on 256 MiB (counter calibration)")
+    return ap.parse_args()
+
+
+def traffic_from_profile():
+    """HBM bytes per launch from the committed rocprofv3 counter summary."""
+    p = ROOT / "profiles" / "c2c_n4096_b4096_pmc.md"
+    if not p.exists():
+        return None
+    m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", p.read_text())
+    return float(m.group(1)) if m else None
+
+
+def cpu_baseline(budget_s: float):
+    """The reference's refFftF32 (FP64 radix-2, one core) over a bounded
+    sample of the same workload: passes over the 4096 seeded rows until the
+    time budget is spent."""
+    import numpy as np
+    import oracle
+    r = oracle.ref()
+    kind = "reference"
+    if r is not None:
+        fn = r.ref_fft_f32
+    else:
+        fn = oracle.lib().orc_ref_fft_f32
+        kind = "port"
+    x = oracle.uniform_input(0, (BATCH, 2 * N))
+    rows_in = [np.ascontiguousarray(x[i]) for i in range(BATCH)]
+    out = np.empty(2 * N, np.float32)
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < budget_s:
+        for i in range(256):
+            fn(rows_in[(done + i) % BATCH], out, N)
+        done += 256
+    dt = time.perf_counter() - t0
+    return {
+        "value": round(16.0 * N * done / dt / 1e9, 4),
+        "unit": "GB/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": f"{done} transforms ({done / BATCH:.2f} passes over the 4096 seeded rows) of "
+                  f"the N=4096 C2C workload through the reference's refFftF32 (FP64, one "
+                  f"thread) in {dt:.1f} s",
+    }
+
+
+def main():
+    args = parse()
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus N>1 is not "
+                                   "supported (batch rows would shard with no communication)"}))
+        sys.exit(2)
+    import numpy as np
+    import torch
+    from bfly import ops, tune
+
+    dev = torch.device("cuda:0")
+    torch.manual_seed(0)
+    pairs = []
+    rng = np.random.default_rng(0)
+    host = rng.uniform(-1.0, 1.0, size=(BATCH, 2 * N)).astype(np.float32)
+    for i in range(max(1, args.rotate)):
+        x = torch.from_numpy(host).to(dev) if i == 0 else torch.roll(pairs[0][0], shifts=i, dims=0)
+        pairs.append((x.contiguous(), torch.empty_like(x)))
+    stream = torch.cuda.current_stream(dev)
+
+    def step(i):
+        x, y = pairs[i % len(pairs)]
+        ops.c2c_raw(x, y, N, BATCH, -1)
+
+    for i in range(args.warmup):
+        step(i)
+    torch.cuda.synchronize()
+    ev0 = torch.cuda.Event(enable_timing=True)
+    ev1 = torch.cuda.Event(enable_timing=True)
+    t0 = time.perf_counter()
+    ev0.record(stream)
+    for i in range(args.steps):
+        step(i)
+    ev1.record(stream)
+    torch.cuda.synchronize()
+    wall = time.perf_counter() - t0
+    kernel_ms = ev0.elapsed_time(ev1) / args.steps
+    ms_per_step = wall * 1e3 / args.steps
+    bytes_alg = 16.0 * N * BATCH
+    value = bytes_alg / (ms_per_step * 1e-3) / 1e9
+    achieved = bytes_alg / (kernel_ms * 1e-3) / 1e9
+
+    # accuracy of the plain path (outside the timed region)
+    x0, y0 = pairs[0]
+    ops.c2c_raw(x0, y0, N, BATCH, -1)
+    torch.cuda.synchronize()
+    sample = np.arange(0, BATCH, 64)
+    got = y0[sample].cpu().numpy()
+    rel = tune.rel_l2_fp64(got, host[sample], N)
+
+    chosen = tune.chosen_vs_plain(N, BATCH, dev, host, sample)
+
+    if args.probe_copy:
+        # Known-size traffic for calibrating FETCH_SIZE / WRITE_SIZE:
+        # 256 MiB read + 256 MiB written per launch.
+        src = torch.empty(64 << 20, dtype=torch.float32, device=dev).uniform_()
+        dst = torch.empty_like(src)
+        for _ in range(3):
+            ops.probe_copy(src, dst)
+        torch.cuda.synchronize()
+
+    line = {
+        "metric": METRIC,
+        "value": round(value, 1),
+        "unit": "GB/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_per_step, 5),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32",
+        "data": "synthetic: U[-1,1) interleaved complex, numpy default_rng(0), rotated over "
+                f"{len(pairs)} resident buffer pairs",
+        "config": {"workload": "c2c_fwd_fp32_n4096_b4096", "n": N, "batch": BATCH,
+                   "layout": "interleaved [batch][2n] float32", "parallelism": "single GPU"},
+        "pct_hbm_peak": round(100.0 * value / HBM_PEAK_GBS, 2),
+        "transforms_per_s": round(BATCH / (ms_per_step * 1e-3), 1),
+        "rel_l2_vs_fp64": rel,
+        "chosen": chosen,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": round(achieved, 1),
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4),
+            "traffic": traffic_from_profile(),
+            "kernel_us": round(kernel_ms * 1e3, 3),
+            "algorithmic_bytes_per_launch": int(bytes_alg),
+        },
+        "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()
