@@ -1,0 +1,146 @@
+"""Host-side checks of the C ABI: both libraries load and export every
+symbol include/bfly_hip.h declares; the host table functions produce the
+FP64 twiddle layout the kernels expect; execute guards through the stub.
+No device compute is called here."""
+import ctypes as C
+import re
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import bfly
+from bfly import _lib
+
+ROOT = Path(__file__).resolve().parents[1]
+HEADER = ROOT / "include" / "bfly_hip.h"
+KINDS = (bfly.KIND_C2C, bfly.KIND_R2C, bfly.KIND_C2R)
+
+
+def declared():
+    return sorted(set(re.findall(r"BFLY_API\s+(?:int|size_t)\s+(bfly_\w+)\s*\(", HEADER.read_text())))
+
+
+def exported(path):
+    out = subprocess.run(["nm", "-D", "--defined-only", str(path)], capture_output=True, text=True,
+                         check=True).stdout
+    return {line.split()[-1] for line in out.splitlines() if " T " in line}
+
+
+def test_header_lists_the_binding_symbols():
+    assert declared() == sorted(_lib.SYMBOLS)
+
+
+@pytest.mark.parametrize("lib", [_lib.HIP_LIB, _lib.STUB_LIB])
+def test_libraries_export_every_declared_symbol(lib):
+    assert lib.exists(), f"{lib} not built"
+    names = exported(lib)
+    missing = [s for s in declared() if s not in names]
+    assert not missing, missing
+    # nothing but the C ABI (and the stub's controls) leaks out
+    extra = {s for s in names if s.startswith("bfly") and s not in declared()}
+    if lib == _lib.HIP_LIB:
+        assert not extra, extra
+
+
+def test_hip_library_loads_without_a_gpu():
+    lib = bfly.hip()  # dlopen only; no device call
+    assert lib.bfly_supported(bfly.KIND_C2C, 4096) == 1
+
+
+def _stage_table(radices):
+    """Closed form of the inter-pass table: pass s >= 1 holds
+    W_{Ns R}^{r k} at [k R + r], Ns = R_0 ... R_{s-1}."""
+    out, ns = [], 1
+    for s, R in enumerate(radices):
+        if s > 0:
+            k = np.arange(ns)[:, None]
+            r = np.arange(R)[None, :]
+            out.append(np.exp(-2j * np.pi * (r * k % (ns * R)) / (ns * R)).reshape(-1))
+        ns *= R
+    return np.concatenate(out) if out else np.zeros(2, complex)
+
+
+PLANS = {16: [16], 32: [16, 2], 64: [16, 4], 128: [16, 8], 256: [16, 16], 512: [16, 16, 2],
+         1024: [16, 16, 4], 2048: [16, 16, 8], 4096: [16, 16, 16]}
+
+
+@pytest.mark.parametrize("n", sorted(PLANS))
+def test_c2c_table_is_fp64_twiddles(n):
+    lib = bfly.hip()
+    nf = lib.bfly_table_floats(bfly.KIND_C2C, n)
+    t = np.empty(nf, np.float32)
+    assert lib.bfly_table_fill(bfly.KIND_C2C, n, t.ctypes.data) == 0
+    want = _stage_table(PLANS[n])
+    got = t[0::2] + 1j * t[1::2]
+    if len(PLANS[n]) == 1:
+        assert nf == 4 and not np.any(t)
+        return
+    assert got.size == want.size
+    # each entry is the FP64 value rounded once to fp32
+    assert np.array_equal(t[0::2], want.real.astype(np.float32))
+    assert np.array_equal(t[1::2], want.imag.astype(np.float32))
+
+
+@pytest.mark.parametrize("n", [8192, 16384, 32768, 65536])
+def test_long_table_layout(n):
+    lib = bfly.hip()
+    t = np.empty(lib.bfly_table_floats(bfly.KIND_C2C, n), np.float32)
+    assert lib.bfly_table_fill(bfly.KIND_C2C, n, t.ctypes.data) == 0
+    n1 = 1 << ((n.bit_length() - 1) // 2)
+    n2 = n // n1
+    s1, s2 = _stage_table(PLANS[n1]), _stage_table(PLANS[n2])
+    c = t[0::2] + 1j * t[1::2]
+    assert np.array_equal(c[:s1.size], s1.astype(np.complex64))
+    assert np.array_equal(c[s1.size:s1.size + s2.size], s2.astype(np.complex64))
+    w = np.exp(-2j * np.pi * np.arange(n) / n).astype(np.complex64)
+    assert np.array_equal(c[s1.size + s2.size:], w)
+    assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 64) == n * 64 * 8
+
+
+@pytest.mark.parametrize("kind", [bfly.KIND_R2C, bfly.KIND_C2R])
+@pytest.mark.parametrize("n", [32, 128, 8192])
+def test_real_table_layout(kind, n):
+    lib = bfly.hip()
+    t = np.empty(lib.bfly_table_floats(kind, n), np.float32)
+    assert lib.bfly_table_fill(kind, n, t.ctypes.data) == 0
+    s = _stage_table(PLANS[n // 2])
+    c = t[0::2] + 1j * t[1::2]
+    off = 0 if len(PLANS[n // 2]) == 1 else s.size
+    w = np.exp(-2j * np.pi * np.arange(n // 2 + 1) / n).astype(np.complex64)
+    assert np.array_equal(c[off:off + n // 2 + 1], w)
+
+
+def test_supported_envelopes():
+    lib = bfly.hip()
+    for n in [1 << k for k in range(4, 17)]:
+        assert lib.bfly_supported(bfly.KIND_C2C, n) == 1
+    for n in (0, 1, 8, 100, 131072, 1 << 20):
+        assert lib.bfly_supported(bfly.KIND_C2C, n) == 0
+    for kind in (bfly.KIND_R2C, bfly.KIND_C2R):
+        assert [lib.bfly_supported(kind, 1 << k) for k in range(3, 16)] == \
+            [0, 0, 1, 1, 1, 1, 1, 1, 1, 1, 1, 0, 0]
+    assert lib.bfly_supported(7, 64) == 0
+    assert lib.bfly_table_floats(bfly.KIND_C2C, 100) == 0
+    assert lib.bfly_table_fill(bfly.KIND_C2C, 100, None) == bfly.ERR_SHAPE
+
+
+def test_execute_guards_through_stub():
+    s = bfly.stub()
+    s.bfly_stub_reset()
+    buf = np.zeros(1 << 16, np.float32)
+    p = buf.ctypes.data
+    assert s.bfly_c2c(p, p, 4096, 1, -1, p, None, None) == 0
+    assert s.bfly_c2c(p, p, 4096, 0, -1, p, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(None, p, 4096, 1, -1, p, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(p, p, 4096, 1, 0, p, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(p, p, 4096, 1, -1, None, None, None) == _lib.ERR_TABLE
+    assert s.bfly_c2c(p, p, 8192, 1, -1, p, None, None) == bfly.ERR_SHAPE  # long needs work
+    assert s.bfly_c2c(p, p, 8192, 1, -1, p, p, None) == 0
+    assert s.bfly_r2c(p, p, 16, 1, p, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2r(p, p, 16384, 1, p, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(p, p, 4096, 0x7FFFFFFF, -1, p, None, None) == 0  # grid fits
+    cnt = (C.c_int * 8)()
+    s.bfly_stub_counters(cnt)
+    assert cnt[3] == 3  # launches counted only for accepted calls
