@@ -25,6 +25,12 @@ ERR_SYNC = -6
 ERR_D2H = -7
 ERR_SHAPE = -8
 
+VARIANT_AUTO = 0
+VARIANT_PLAIN = 1
+VARIANT_STREAM = 2
+VARIANT_PLAIN_SWZ = 3
+VARIANT_STREAM_SWZ = 4
+
 KIND_C2C = 0
 KIND_R2C = 1
 KIND_C2R = 2
@@ -45,7 +51,8 @@ SYMBOLS = (
     "bfly_rt_init", "bfly_rt_stream_create", "bfly_rt_stream_destroy", "bfly_rt_stream_sync",
     "bfly_rt_malloc", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
     "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
-    "bfly_workspace_bytes", "bfly_c2c", "bfly_r2c", "bfly_c2r", "bfly_probe_copy",
+    "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
+    "bfly_c2r_ex", "bfly_probe_copy",
     "bfly_probe_device",
 )
 
@@ -74,8 +81,11 @@ def _bind(lib):
         "bfly_table_fill": (i32, [i32, u32, vp]),
         "bfly_workspace_bytes": (sz, [i32, u32, u32]),
         "bfly_c2c": (i32, [vp, vp, u32, u32, i32, vp, vp, vp]),
+        "bfly_c2c_ex": (i32, [vp, vp, u32, u32, i32, i32, vp, vp, vp]),
         "bfly_r2c": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),
+        "bfly_r2c_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
+        "bfly_c2r_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
         "bfly_probe_copy": (i32, [vp, vp, sz, vp]),
         "bfly_probe_device": (i32, [i32, C.c_char_p, sz]),
     }

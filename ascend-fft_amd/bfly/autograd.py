@@ -1,0 +1,143 @@
+"""torch.autograd.Functions over the gfx950 FFT kernels, with adjoint FFTs.
+
+Adjoints (unnormalised forward F, conjugate-Wirtinger gradients as torch uses):
+  c2c(x, s)     y = F_s x              grad_x = F_{-s} grad_y          (F^H = F_{-s})
+  rfft(x)       y = (F x)[0..n/2]      grad_x = n * irfft(w . grad_y), w = 1 at bins 0 and n/2,
+                                       1/2 elsewhere (the Hermitian weights of the real input)
+  irfft(h, n)   x = irfft(h)           grad_h = rfft(grad_x) * v / n, v = 1 at bins 0 and n/2,
+                                       2 elsewhere, imaginary parts of bins 0 and n/2 zero
+Every adjoint is itself one launch of the library's kernels (no torch.fft call).
+"""
+from __future__ import annotations
+
+import torch
+
+from . import ops
+
+
+def _as_interleaved(z: torch.Tensor) -> torch.Tensor:
+    return torch.view_as_real(z.contiguous()).reshape(*z.shape[:-1], 2 * z.shape[-1])
+
+
+def _as_complex(f: torch.Tensor) -> torch.Tensor:
+    return torch.view_as_complex(f.reshape(*f.shape[:-1], f.shape[-1] // 2, 2))
+
+
+class C2C(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, x: torch.Tensor, sign: int = -1):
+        ctx.sign = sign
+        return ops.c2c(x.contiguous(), sign)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        return ops.c2c(g.contiguous(), -ctx.sign), None
+
+
+class RFFT(torch.autograd.Function):
+    """float32 [..., n] -> complex64 [..., n/2+1]."""
+
+    @staticmethod
+    def forward(ctx, x: torch.Tensor):
+        n = x.shape[-1]
+        ctx.n = n
+        h = ops.rfft(x.contiguous())
+        return _as_complex(h)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        n = ctx.n
+        w = torch.full((n // 2 + 1,), 0.5, dtype=torch.float32, device=g.device)
+        w[0] = 1.0
+        w[-1] = 1.0
+        gw = _as_interleaved(g * w)
+        return ops.irfft(gw, n) * n
+
+
+class IRFFT(torch.autograd.Function):
+    """complex64 [..., n/2+1] -> float32 [..., n] (1/n)."""
+
+    @staticmethod
+    def forward(ctx, h: torch.Tensor, n: int):
+        ctx.n = n
+        return ops.irfft(_as_interleaved(h), n)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        n = ctx.n
+        z = _as_complex(ops.rfft(g.contiguous()))
+        v = torch.full((n // 2 + 1,), 2.0 / n, dtype=torch.float32, device=g.device)
+        v[0] = 1.0 / n
+        v[-1] = 1.0 / n
+        z = z * v
+        # Bins 0 and n/2 of irfft only read their real parts.
+        z[..., 0] = z[..., 0].real.to(z.dtype)
+        z[..., -1] = z[..., -1].real.to(z.dtype)
+        return z, None
+
+
+def _modes_weights(n: int, modes: int, interior: float, edge: float, device) -> torch.Tensor:
+    w = torch.full((modes,), interior, dtype=torch.float32, device=device)
+    w[0] = edge
+    if modes == n // 2 + 1:
+        w[-1] = edge
+    return w
+
+
+class RFFTModes(torch.autograd.Function):
+    """float32 [..., n] -> complex64 [..., K]: the first K bins of the R2C."""
+
+    @staticmethod
+    def forward(ctx, x: torch.Tensor, modes: int):
+        ctx.n, ctx.modes = x.shape[-1], modes
+        return _as_complex(ops.rfft_modes(x.contiguous(), modes))
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        n, K = ctx.n, ctx.modes
+        gw = _as_interleaved(g * _modes_weights(n, K, 0.5, 1.0, g.device))
+        return ops.irfft_modes(gw, n) * n, None
+
+
+class IRFFTModes(torch.autograd.Function):
+    """complex64 [..., K] (higher bins zero) -> float32 [..., n] (1/n)."""
+
+    @staticmethod
+    def forward(ctx, h: torch.Tensor, n: int):
+        ctx.n, ctx.modes = n, h.shape[-1]
+        return ops.irfft_modes(_as_interleaved(h), n)
+
+    @staticmethod
+    def backward(ctx, g: torch.Tensor):
+        n, K = ctx.n, ctx.modes
+        z = _as_complex(ops.rfft_modes(g.contiguous(), K))
+        z = z * _modes_weights(n, K, 2.0 / n, 1.0 / n, g.device)
+        z[..., 0] = z[..., 0].real.to(z.dtype)
+        if K == n // 2 + 1:
+            z[..., -1] = z[..., -1].real.to(z.dtype)
+        return z, None
+
+
+def rfft_modes(x: torch.Tensor, modes: int) -> torch.Tensor:
+    return RFFTModes.apply(x, modes)
+
+
+def irfft_modes(h: torch.Tensor, n: int) -> torch.Tensor:
+    return IRFFTModes.apply(h, n)
+
+
+def fft(x: torch.Tensor) -> torch.Tensor:
+    """Differentiable forward C2C of complex64 [..., n]."""
+    return C2C.apply(x, -1)
+
+
+def ifft_unnormalized(x: torch.Tensor) -> torch.Tensor:
+    return C2C.apply(x, 1)
+
+
+def rfft(x: torch.Tensor) -> torch.Tensor:
+    return RFFT.apply(x)
+
+
+def irfft(h: torch.Tensor, n: int) -> torch.Tensor:
+    return IRFFT.apply(h, n)

@@ -121,6 +121,38 @@ def irfft(h: torch.Tensor, n: int) -> torch.Tensor:
     return out
 
 
+def rfft_modes(x: torch.Tensor, modes: int) -> torch.Tensor:
+    """First `modes` bins of the R2C along the last axis: float32 [..., n] ->
+    float32 [..., 2*modes] (interleaved); the other bins are never written."""
+    _check_device(x, "x")
+    n = x.shape[-1]
+    batch = x.numel() // n if n else 0
+    out = torch.empty(*x.shape[:-1], 2 * modes, dtype=torch.float32, device=x.device)
+    if batch == 0:
+        return out
+    tab = table(_lib.KIND_R2C, n, x.device)
+    rc = _lib.hip().bfly_r2c_ex(x.data_ptr(), out.data_ptr(), n, batch, modes, 2 * modes,
+                                tab.data_ptr(), _stream(x.device))
+    _lib.check(rc, f"bfly_r2c_ex(n={n}, batch={batch}, modes={modes})")
+    return out
+
+
+def irfft_modes(h: torch.Tensor, n: int) -> torch.Tensor:
+    """C2R of a spectrum given by its first modes bins (float32 [..., 2*modes],
+    interleaved; every higher bin zero) -> float32 [..., n], 1/n."""
+    _check_device(h, "h")
+    modes = h.shape[-1] // 2
+    batch = h.numel() // (2 * modes)
+    out = torch.empty(*h.shape[:-1], n, dtype=torch.float32, device=h.device)
+    if batch == 0:
+        return out
+    tab = table(_lib.KIND_C2R, n, h.device)
+    rc = _lib.hip().bfly_c2r_ex(h.data_ptr(), out.data_ptr(), n, batch, modes, 2 * modes,
+                                tab.data_ptr(), _stream(h.device))
+    _lib.check(rc, f"bfly_c2r_ex(n={n}, batch={batch}, modes={modes})")
+    return out
+
+
 def probe_copy(src: torch.Tensor, dst: torch.Tensor) -> None:
     rc = _lib.hip().bfly_probe_copy(src.data_ptr(), dst.data_ptr(), src.numel() * src.element_size(),
                                     _stream(src.device))

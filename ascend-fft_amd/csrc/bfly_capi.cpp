@@ -74,22 +74,41 @@ int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
     return hipStreamSynchronize(S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_D2H;
 }
 
+int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
+                const float* table, void* work, void* stream) {
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return g;
+    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_variant(n, batch);
+    return launched(bfly_kern::c2c(in, out, n, batch, sign, variant, table, work, S(stream)));
+}
+
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream) {
-    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return g;
-    return launched(bfly_kern::c2c(in, out, n, batch, sign, table, work, S(stream)));
+    return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);
+}
+
+int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                uint32_t out_stride, const float* table, void* stream) {
+    if (int g = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return g;
+    if (!bfly_host::bins_ok(n, bins, out_stride)) return BFLY_ERR_SHAPE;
+    return launched(bfly_kern::real(BFLY_KIND_R2C, in, out, n, batch, bins, out_stride, table, S(stream)));
 }
 
 int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
              void* stream) {
-    if (int g = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return g;
-    return launched(bfly_kern::real(BFLY_KIND_R2C, in, out, n, batch, table, S(stream)));
+    return bfly_r2c_ex(in, out, n, batch, n / 2 + 1, n + 2, table, stream);
+}
+
+int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                uint32_t in_stride, const float* table, void* stream) {
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return g;
+    if (!bfly_host::bins_ok(n, bins, in_stride)) return BFLY_ERR_SHAPE;
+    return launched(bfly_kern::real(BFLY_KIND_C2R, in, out, n, batch, bins, in_stride, table, S(stream)));
 }
 
 int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
              void* stream) {
-    if (int g = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return g;
-    return launched(bfly_kern::real(BFLY_KIND_C2R, in, out, n, batch, table, S(stream)));
+    return bfly_c2r_ex(in, out, n, batch, n / 2 + 1, n + 2, table, stream);
 }
 
 int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,

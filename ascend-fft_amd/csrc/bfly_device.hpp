@@ -18,6 +18,37 @@
 
 namespace bfly_dev {
 
+// ------------------------------------------------------- streaming access
+// Transform data is touched once per launch: non-temporal loads and stores
+// keep it from displacing the twiddle tables in L2 and let the memory
+// controller stream it (measured on MI355X: a 4-deep float4 copy goes from
+// 5.6 to 6.2 TB/s with non-temporal access).
+typedef float bfly_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 ld_stream(const float2* p) {
+    const bfly_v2f v = __builtin_nontemporal_load(reinterpret_cast<const bfly_v2f*>(p));
+    return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ void st_stream(float2* p, float2 x) {
+    bfly_v2f v;
+    v.x = x.x;
+    v.y = x.y;
+    __builtin_nontemporal_store(v, reinterpret_cast<bfly_v2f*>(p));
+}
+
+// Streaming access only pays when a row segment covers whole 128-byte lines
+// (16 consecutive complex values or more); shorter segments rely on L2
+// write-combining, which non-temporal stores bypass.
+template <bool NT>
+__device__ __forceinline__ float2 ld_io(const float2* p) {
+    if constexpr (NT) return ld_stream(p);
+    else return *p;
+}
+template <bool NT>
+__device__ __forceinline__ void st_io(float2* p, float2 x) {
+    if constexpr (NT) st_stream(p, x);
+    else *p = x;
+}
+
 // ------------------------------------------------------------ complex ops
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -146,6 +177,64 @@ constexpr int PAD_SHIFT = 4;
 __device__ __forceinline__ int lds_idx(int i) { return i + (i >> PAD_SHIFT); }
 __host__ __device__ constexpr int lds_len(int n) { return n + (n >> PAD_SHIFT); }
 
+// Two exchange layouts.  LX_PAD: region pointer + padded local index (one
+// slot per 16 complex).  LX_SWZ: no padding; complex index i of the whole
+// workgroup buffer lives at i ^ ((i >> 5) & 31), a bijection that spreads
+// the stride-16 and stride-256 Stockham writes over all 64 banks (each
+// 8-byte slot owns two of them) while consecutive reads stay conflict-free.
+// Without padding a 256-thread workgroup needs exactly 32 KiB, so five fit
+// the 160 KiB of LDS of a CU instead of four.
+constexpr int LX_PAD = 0;
+constexpr int LX_SWZ = 1;
+template <int LX>
+__device__ __forceinline__ int lx(int rb, int i) {
+    if constexpr (LX == LX_SWZ) {
+        const int g = rb + i;
+        return g ^ ((g >> 5) & 31);
+    } else {
+        return lds_idx(i);
+    }
+}
+template <int LX>
+__host__ __device__ constexpr int lx_len(int n) { return LX == LX_SWZ ? n : lds_len(n); }
+
+// Address of element r of a Stockham write group (base + r*Ns) and of a read
+// group (u + r*S).  In the padded layout every offset r*Ns (Ns >= 16) or
+// r*S (S >= 16) is a multiple of 16, so pad(x + c) = pad(x) + c*17/16 and
+// the whole group is one per-thread base plus compile-time offsets (LDS
+// immediates); for Ns == 1 the group never crosses a 16-slot boundary
+// (R | 16), so pad(base + r) = pad(base) + r.
+template <int LX, int Ns>
+__device__ __forceinline__ int lx_wbase(int rb, int base) {
+    if constexpr (LX == LX_PAD) return lds_idx(base);
+    else return rb + base;
+}
+template <int LX, int Ns>
+__device__ __forceinline__ int lx_wat(int wb, int r) {
+    if constexpr (LX == LX_PAD) {
+        static_assert(Ns == 1 || Ns % 16 == 0, "pass offsets must be multiples of the pad period");
+        return Ns == 1 ? wb + r : wb + r * Ns + (r * Ns >> PAD_SHIFT);
+    } else {
+        const int g = wb + r * Ns;
+        return g ^ ((g >> 5) & 31);
+    }
+}
+template <int LX, int S>
+__device__ __forceinline__ int lx_rat(int rbase, int r) {
+    if constexpr (LX == LX_PAD) {
+        static_assert(S % 16 == 0, "read stride must be a multiple of the pad period");
+        return rbase + r * S + (r * S >> PAD_SHIFT);
+    } else {
+        const int g = rbase + r * S;
+        return g ^ ((g >> 5) & 31);
+    }
+}
+template <int LX>
+__device__ __forceinline__ int lx_rbase(int rb, int u) {
+    if constexpr (LX == LX_PAD) return lds_idx(u);
+    else return rb + u;
+}
+
 // ------------------------------------------------------- Stockham passes
 // Radix plans: the list of radices of one length, first pass first.
 template <int... R>
@@ -178,9 +267,9 @@ struct First { static constexpr int value = R0; };
 // on exit v[m*RL + r] = X[t + m*T + r*N/RL] for the last radix RL.
 // `lds` is this transform's region (lds_len(N) complex); `tw` the table of
 // the current pass (unused for the first).
-template <int N, int P, int T, int DIR, int Ns, int R, int... Rest>
-__device__ __forceinline__ void stockham(float2 (&v)[P], float2* lds, const float2* __restrict__ tw,
-                                         int t) {
+template <int N, int P, int T, int DIR, int LX, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stockham(float2 (&v)[P], float2* lds, int rb,
+                                         const float2* __restrict__ tw, int t) {
     static_assert(P % R == 0, "radix must divide the points per thread");
     constexpr int M = P / R;
     if constexpr (Ns > 1) {
@@ -203,19 +292,145 @@ __device__ __forceinline__ void stockham(float2 (&v)[P], float2* lds, const floa
         for (int m = 0; m < M; m++) {
             const int j = t + m * T;
             const int k = j & (Ns - 1);
-            const int base = (j - k) * R + k;
+            const int wb = lx_wbase<LX, Ns>(rb, (j - k) * R + k);
 #pragma unroll
-            for (int r = 0; r < R; r++) lds[lds_idx(base + r * Ns)] = v[m * R + r];
+            for (int r = 0; r < R; r++) lds[lx_wat<LX, Ns>(wb, r)] = v[m * R + r];
         }
         __syncthreads();
         constexpr int R2 = First<Rest...>::value;
         constexpr int M2 = P / R2;
 #pragma unroll
-        for (int m = 0; m < M2; m++)
+        for (int m = 0; m < M2; m++) {
+            const int rbase = lx_rbase<LX>(rb, t + m * T);
 #pragma unroll
-            for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lds_idx(t + m * T + r * (N / R2))];
+            for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
+        }
         __syncthreads();
-        stockham<N, P, T, DIR, Ns * R, Rest...>(v, lds, tw + (Ns > 1 ? Ns * R : 0), t);
+        stockham<N, P, T, DIR, LX, Ns * R, Rest...>(v, lds, rb, tw + (Ns > 1 ? Ns * R : 0), t);
+    }
+}
+
+// ------------------------------------------- register-resident twiddles
+// The twiddles a thread multiplies by depend only on its index t inside the
+// transform (k = (t + m T) mod Ns), never on which transform it works on.
+// A thread that processes a stream of transforms loads them once into
+// registers, so the passes issue no global loads at all -- which lets the
+// next transform's input stay in flight across the whole compute phase.
+template <int P, int Ns, int... R>
+struct TwRegCount;
+template <int P, int Ns>
+struct TwRegCount<P, Ns> { static constexpr int value = 0; };
+template <int P, int Ns, int R0, int... R>
+struct TwRegCount<P, Ns, R0, R...> {
+    static constexpr int value = (Ns > 1 ? (P / R0) * (R0 - 1) : 0) + TwRegCount<P, Ns * R0, R...>::value;
+};
+
+// Fill tr[] with every twiddle of passes s >= 1 in pass order, [m][r-1].
+template <int N, int P, int T, int Ns, int R, int... Rest>
+__device__ __forceinline__ void load_twiddles(float2* tr, const float2* __restrict__ tw, int t) {
+    constexpr int M = P / R;
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int k = (t + m * T) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; r++) tr[m * (R - 1) + r - 1] = tw[k * R + r];
+        }
+    }
+    if constexpr (sizeof...(Rest) > 0)
+        load_twiddles<N, P, T, Ns * R, Rest...>(tr + (Ns > 1 ? M * (R - 1) : 0),
+                                                tw + (Ns > 1 ? Ns * R : 0), t);
+}
+
+// Same passes as stockham(), twiddles taken from registers.
+template <int N, int P, int T, int DIR, int LX, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stockham_rt(float2 (&v)[P], float2* lds, int rb, const float2* tr,
+                                            int t) {
+    constexpr int M = P / R;
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int m = 0; m < M; m++)
+#pragma unroll
+            for (int r = 1; r < R; r++)
+                v[m * R + r] = cmul_tw<DIR>(v[m * R + r], tr[m * (R - 1) + r - 1]);
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) dft<R, DIR>(&v[m * R]);
+    if constexpr (sizeof...(Rest) > 0) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int j = t + m * T;
+            const int k = j & (Ns - 1);
+            const int wb = lx_wbase<LX, Ns>(rb, (j - k) * R + k);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lx_wat<LX, Ns>(wb, r)] = v[m * R + r];
+        }
+        __syncthreads();
+        constexpr int R2 = First<Rest...>::value;
+        constexpr int M2 = P / R2;
+#pragma unroll
+        for (int m = 0; m < M2; m++) {
+            const int rbase = lx_rbase<LX>(rb, t + m * T);
+#pragma unroll
+            for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
+        }
+        __syncthreads();
+        stockham_rt<N, P, T, DIR, LX, Ns * R, Rest...>(v, lds, rb, tr + (Ns > 1 ? M * (R - 1) : 0), t);
+    }
+}
+
+// ------------------------------------------- two-level LDS twiddles
+// W_N^m = A[m mod 64] * B[m / 64] with A[j] = W_N^j (j < 64) and
+// B[j] = W_N^{64 j} (j < N/64), both FP64-generated and held in LDS (at most
+// 128 complex).  One extra complex product per twiddle replaces the
+// register-resident twiddle file of stockham_rt, which frees the registers
+// for more resident workgroups; the product adds about one ulp.
+constexpr int TW2_LO = 64;
+template <int N>
+__device__ __forceinline__ float2 tw2(const float2* tlo, const float2* thi, int m) {
+    if constexpr (N <= TW2_LO) {
+        return tlo[m];
+    } else {
+        const float2 a = tlo[m & (TW2_LO - 1)], b = thi[m / TW2_LO];
+        return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+}
+
+template <int N, int P, int T, int DIR, int LX, int Ns, int R, int... Rest>
+__device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb, const float2* tlo,
+                                            const float2* thi, int t) {
+    constexpr int M = P / R;
+    if constexpr (Ns > 1) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int k = (t + m * T) & (Ns - 1);
+#pragma unroll
+            for (int r = 1; r < R; r++)
+                v[m * R + r] = cmul_tw<DIR>(v[m * R + r], tw2<N>(tlo, thi, r * k * (N / (Ns * R))));
+        }
+    }
+#pragma unroll
+    for (int m = 0; m < M; m++) dft<R, DIR>(&v[m * R]);
+    if constexpr (sizeof...(Rest) > 0) {
+#pragma unroll
+        for (int m = 0; m < M; m++) {
+            const int j = t + m * T;
+            const int k = j & (Ns - 1);
+            const int wb = lx_wbase<LX, Ns>(rb, (j - k) * R + k);
+#pragma unroll
+            for (int r = 0; r < R; r++) lds[lx_wat<LX, Ns>(wb, r)] = v[m * R + r];
+        }
+        __syncthreads();
+        constexpr int R2 = First<Rest...>::value;
+        constexpr int M2 = P / R2;
+#pragma unroll
+        for (int m = 0; m < M2; m++) {
+            const int rbase = lx_rbase<LX>(rb, t + m * T);
+#pragma unroll
+            for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
+        }
+        __syncthreads();
+        stockham_lt<N, P, T, DIR, LX, Ns * R, Rest...>(v, lds, rb, tlo, thi, t);
     }
 }
 

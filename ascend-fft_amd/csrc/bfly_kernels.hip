@@ -16,6 +16,7 @@
 #include <utility>
 
 #include "bfly_device.hpp"
+#include "bfly_hip.h"
 #include "bfly_kernels.hpp"
 #include "bfly_plans.hpp"
 #include "bfly_tables.hpp"
@@ -35,28 +36,54 @@ struct PlanOf {
     static constexpr int T = N / P;  // threads per transform
 };
 
-template <int I, int T, int DIR, size_t... S>
-__device__ __forceinline__ void run_plan(float2 (&v)[bfly_plan::kPoints], float2* lds,
+template <int I, int T, int DIR, int LX, size_t... S>
+__device__ __forceinline__ void run_plan(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
                                          const float2* __restrict__ tw, int t,
                                          std::index_sequence<S...>) {
-    stockham<kPlans[I].n, bfly_plan::kPoints, T, DIR, 1, kPlans[I].rad[S]...>(v, lds, tw, t);
+    stockham<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tw, t);
 }
 
+template <int I, int T, int DIR, int LX, size_t... S>
+__device__ __forceinline__ void run_plan_rt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
+                                            const float2* tr, int t, std::index_sequence<S...>) {
+    stockham_rt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tr, t);
+}
+
+template <int I, int T, int DIR, int LX, size_t... S>
+__device__ __forceinline__ void run_plan_lt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
+                                            const float2* tlo, const float2* thi, int t,
+                                            std::index_sequence<S...>) {
+    stockham_lt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tlo, thi, t);
+}
+
+template <int I, int T, size_t... S>
+__device__ __forceinline__ void load_plan_twiddles(float2* tr, const float2* __restrict__ tw, int t,
+                                                   std::index_sequence<S...>) {
+    load_twiddles<kPlans[I].n, bfly_plan::kPoints, T, 1, kPlans[I].rad[S]...>(tr, tw, t);
+}
+
+template <int I, size_t... S>
+constexpr int plan_twiddle_regs(std::index_sequence<S...>) {
+    return TwRegCount<bfly_plan::kPoints, 1, kPlans[I].rad[S]...>::value;
+}
+
+// Padded-region form used by the real and long kernels: `lds` is the
+// transform's own region.
 template <int I, int T, int DIR>
 __device__ __forceinline__ void fft_local(float2 (&v)[bfly_plan::kPoints], float2* lds,
                                           const float2* __restrict__ tw, int t) {
-    run_plan<I, T, DIR>(v, lds, tw, t, std::make_index_sequence<kPlans[I].nrad>{});
+    run_plan<I, T, DIR, LX_PAD>(v, lds, 0, tw, t, std::make_index_sequence<kPlans[I].nrad>{});
 }
 
 // ------------------------------------------------------------------ C2C
-template <int I, int DIR>
+template <int I, int DIR, int LX>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
     uint32_t batch) {
     using PL = PlanOf<I>;
     constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    __shared__ float2 lds[B * lds_len(N)];
+    __shared__ float2 lds[B * lx_len<LX>(N)];
     const int tid = threadIdx.x;
     const int bl = tid / T, t = tid % T;
     const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
@@ -67,18 +94,180 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
 #pragma unroll
         for (int m = 0; m < P / R0; m++)
 #pragma unroll
-            for (int r = 0; r < R0; r++) v[m * R0 + r] = src[t + m * T + r * (N / R0)];
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]);
     } else {
 #pragma unroll
         for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
     }
-    fft_local<I, T, DIR>(v, lds + bl * lds_len(N), tw, t);
+    run_plan<I, T, DIR, LX>(v, LX == LX_SWZ ? lds : lds + bl * lds_len(N), LX == LX_SWZ ? bl * N : 0,
+                            tw, t, std::make_index_sequence<PL::NRAD>{});
     if (active) {
         float2* dst = out + tr * N;
 #pragma unroll
         for (int m = 0; m < P / RL; m++)
 #pragma unroll
-            for (int r = 0; r < RL; r++) dst[t + m * T + r * (N / RL)] = v[m * RL + r];
+            for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], v[m * RL + r]);
+    }
+}
+
+// C2C, streaming form: a grid sized to the resident-workgroup capacity walks
+// the batch; each thread keeps its twiddles in registers and issues the loads
+// of its next transform before computing the current one, so HBM reads stay
+// in flight through the LDS exchanges and the register butterflies.
+template <int I, int DIR, int LX>
+__global__ __launch_bounds__(bfly_plan::kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_c2c_stream(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+    uint32_t batch) {
+    using PL = PlanOf<I>;
+    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int TWR = plan_twiddle_regs<I>(std::make_index_sequence<PL::NRAD>{});
+    __shared__ float2 lds[B * lx_len<LX>(N)];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    float2 tr[TWR > 0 ? TWR : 1];
+    load_plan_twiddles<I, T>(tr, tw, t, std::make_index_sequence<PL::NRAD>{});
+    float2 cur[P], nxt[P];
+    uint32_t g = blockIdx.x;
+    if (g < groups) {
+        const uint64_t row = (uint64_t)g * B + bl;
+        const float2* src = in + row * N;
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++)
+                cur[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+    }
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t gn = g + gridDim.x;
+        if (gn < groups) {
+            const uint64_t row = (uint64_t)gn * B + bl;
+            const float2* src = in + row * N;
+#pragma unroll
+            for (int m = 0; m < P / R0; m++)
+#pragma unroll
+                for (int r = 0; r < R0; r++)
+                    nxt[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+        }
+        run_plan_rt<I, T, DIR, LX>(cur, LX == LX_SWZ ? lds : lds + bl * lds_len(N),
+                                   LX == LX_SWZ ? bl * N : 0, tr, t,
+                                   std::make_index_sequence<PL::NRAD>{});
+        const uint64_t row = (uint64_t)g * B + bl;
+        if (row < batch) {
+            float2* dst = out + row * N;
+#pragma unroll
+            for (int m = 0; m < P / RL; m++)
+#pragma unroll
+                for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], cur[m * RL + r]);
+        }
+#pragma unroll
+        for (int i = 0; i < P; i++) cur[i] = nxt[i];
+    }
+}
+
+// Streaming form with two transforms in flight per workgroup: registers
+// hold the transform being computed plus the next two being loaded.
+template <int I, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream2(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+    uint32_t batch) {
+    using PL = PlanOf<I>;
+    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int TWR = plan_twiddle_regs<I>(std::make_index_sequence<PL::NRAD>{});
+    __shared__ float2 lds[B * N];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    float2 tr[TWR > 0 ? TWR : 1];
+    load_plan_twiddles<I, T>(tr, tw, t, std::make_index_sequence<PL::NRAD>{});
+    float2 cur[P], n1[P], n2[P];
+    auto fetch = [&](float2 (&v)[P], uint32_t grp) {
+        const uint64_t row = (uint64_t)grp * B + bl;
+        const float2* src = in + row * N;
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++)
+                v[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+    };
+    uint32_t g = blockIdx.x;
+    if (g < groups) fetch(cur, g);
+    if (g + gridDim.x < groups) fetch(n1, g + gridDim.x);
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t g2 = g + 2 * gridDim.x;
+        if (g2 < groups) fetch(n2, g2);
+        run_plan_rt<I, T, DIR, LX_SWZ>(cur, lds, bl * N, tr, t, std::make_index_sequence<PL::NRAD>{});
+        const uint64_t row = (uint64_t)g * B + bl;
+        if (row < batch) {
+            float2* dst = out + row * N;
+#pragma unroll
+            for (int m = 0; m < P / RL; m++)
+#pragma unroll
+                for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], cur[m * RL + r]);
+        }
+#pragma unroll
+        for (int i = 0; i < P; i++) {
+            cur[i] = n1[i];
+            n1[i] = n2[i];
+        }
+    }
+}
+
+// Streaming form with the two-level twiddle tables in LDS (no twiddle
+// registers): more workgroups stay resident, so more transforms are in
+// flight per CU.  `tw2g` points at [A (64) | B (N/64)] in the table.
+template <int I, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw2g,
+    uint32_t batch) {
+    using PL = PlanOf<I>;
+    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int NLO = N < TW2_LO ? N : TW2_LO;
+    constexpr int NHI = N / NLO;
+    __shared__ float2 lds[B * lds_len(N)];
+    __shared__ float2 ltw[NLO + NHI];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    for (int i = tid; i < NLO + NHI; i += bfly_plan::kBlock) ltw[i] = tw2g[i];
+    __syncthreads();
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    float2 cur[P], nxt[P];
+    uint32_t g = blockIdx.x;
+    if (g < groups) {
+        const uint64_t row = (uint64_t)g * B + bl;
+        const float2* src = in + row * N;
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++)
+                cur[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+    }
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t gn = g + gridDim.x;
+        if (gn < groups) {
+            const uint64_t row = (uint64_t)gn * B + bl;
+            const float2* src = in + row * N;
+#pragma unroll
+            for (int m = 0; m < P / R0; m++)
+#pragma unroll
+                for (int r = 0; r < R0; r++)
+                    nxt[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+        }
+        run_plan_lt<I, T, DIR, LX_PAD>(cur, lds + bl * lds_len(N), 0, ltw, ltw + NLO, t,
+                                       std::make_index_sequence<PL::NRAD>{});
+        const uint64_t row = (uint64_t)g * B + bl;
+        if (row < batch) {
+            float2* dst = out + row * N;
+#pragma unroll
+            for (int m = 0; m < P / RL; m++)
+#pragma unroll
+                for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], cur[m * RL + r]);
+        }
+#pragma unroll
+        for (int i = 0; i < P; i++) cur[i] = nxt[i];
     }
 }
 
@@ -86,12 +275,16 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
 // z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
 //   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
 //   X[k] = E + W_n^k O            (Z[M] == Z[0]).
+// Writes bins k < kout of each row at a row stride of ostride floats (the
+// full transform: kout = M+1, ostride = 2M+2; a mode-truncated one writes
+// only its first kout bins).
 template <int I>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restrict__ in,
                                                            float* __restrict__ out,
                                                            const float2* __restrict__ tw,
                                                            const float2* __restrict__ wn,
-                                                           uint32_t batch) {
+                                                           uint32_t batch, uint32_t kout,
+                                                           uint32_t ostride) {
     using PL = PlanOf<I>;
     constexpr int M = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -107,7 +300,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
 #pragma unroll
         for (int m = 0; m < P / R0; m++)
 #pragma unroll
-            for (int r = 0; r < R0; r++) v[m * R0 + r] = src[t + m * T + r * (M / R0)];
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (M / R0)]);
     } else {
 #pragma unroll
         for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
@@ -121,11 +314,11 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
         for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
     __syncthreads();
     if (!active) return;
-    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M + 2));
+    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
 #pragma unroll
     for (int i = 0; i <= P; i++) {
         const int k = t + i * T;
-        if (k <= M) {
+        if (k <= M && k < (int)kout) {
             const float2 a = L[lds_idx(k & (M - 1))];
             const float2 b = L[lds_idx((M - k) & (M - 1))];
             const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
@@ -133,7 +326,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
             const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
             const float2 w = wn[k];
             const float2 wo = cmul_tw<-1>(o, w);
-            dst[k] = cadd(e, wo);
+            st_io<(T >= 16)>(&dst[k], cadd(e, wo));
         }
     }
 }
@@ -141,12 +334,15 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
 // ------------------------------------------------------------------ C2R
 // Z'[k] = (X[k] + conj X[M-k]) + i W_n^{-k} (X[k] - conj X[M-k]), k < M,
 // with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
+// Reads bins k < kin of each row at a row stride of istride floats; bins at
+// or above kin are zero (the full transform: kin = M+1, istride = 2M+2).
 template <int I>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restrict__ in,
                                                            float* __restrict__ out,
                                                            const float2* __restrict__ tw,
                                                            const float2* __restrict__ wn,
-                                                           uint32_t batch, float scale) {
+                                                           uint32_t batch, float scale, uint32_t kin,
+                                                           uint32_t istride) {
     using PL = PlanOf<I>;
     constexpr int M = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -158,11 +354,11 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restri
     const bool active = tr < batch;
     float2* L = lds + bl * lds_len(M);
     if (active) {
-        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)(2 * M + 2));
+        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)istride);
 #pragma unroll
         for (int i = 0; i <= P; i++) {
             const int k = t + i * T;
-            if (k <= M) L[k] = src[k];
+            if (k <= M) L[k] = k < (int)kin ? ld_io<(T >= 16)>(&src[k]) : make_float2(0.f, 0.f);
         }
     }
     __syncthreads();
@@ -188,7 +384,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restri
 #pragma unroll
         for (int r = 0; r < RL; r++) {
             const float2 z = v[m * RL + r];
-            dst[t + m * T + r * (M / RL)] = make_float2(z.x * scale, z.y * scale);
+            st_io<(T >= 16)>(&dst[t + m * T + r * (M / RL)], make_float2(z.x * scale, z.y * scale));
         }
 }
 
@@ -221,7 +417,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
     for (int m = 0; m < P / R0; m++)
 #pragma unroll
         for (int r = 0; r < R0; r++)
-            v[m * R0 + r] = src[(uint64_t)N2 * (t + m * T + r * (N1 / R0))];
+            v[m * R0 + r] = ld_stream(&src[(uint64_t)N2 * (t + m * T + r * (N1 / R0))]);
     fft_local<I1, T, DIR>(v, lds + c * STRIDE, tw1, t);
     float2* dst = ws + b * N + n2;
 #pragma unroll
@@ -229,7 +425,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
 #pragma unroll
         for (int r = 0; r < RL; r++) {
             const int k1 = t + m * T + r * (N1 / RL);
-            dst[(uint64_t)N2 * k1] = cmul_tw<DIR>(v[m * RL + r], wN[n2 * k1]);
+            st_stream(&dst[(uint64_t)N2 * k1], cmul_tw<DIR>(v[m * RL + r], wN[n2 * k1]));
         }
 }
 
@@ -253,7 +449,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_rows(
     for (int i = 0; i < C * N2 / bfly_plan::kBlock; i++) {
         const int idx = tid + i * bfly_plan::kBlock;
         const int row = idx / N2, col = idx % N2;
-        lds[row * STRIDE + lds_idx(col)] = src[idx];
+        lds[row * STRIDE + lds_idx(col)] = ld_stream(&src[idx]);
     }
     __syncthreads();
     const int c = tid % C, t = tid / C;
@@ -271,7 +467,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_rows(
 #pragma unroll
         for (int r = 0; r < RL; r++) {
             const int k2 = t + m * T + r * (N2 / RL);
-            dst[(uint64_t)N1 * k2] = v[m * RL + r];
+            st_stream(&dst[(uint64_t)N1 * k2], v[m * RL + r]);
         }
 }
 
@@ -284,28 +480,91 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
 }
 
 // ------------------------------------------------------------ dispatch
-template <int I>
-hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uint32_t batch,
-                            int sign, hipStream_t s) {
+// Resident workgroups of kernel f on the current device (CUs x occupancy).
+int resident_blocks(const void* f) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, bfly_plan::kBlock, 0) != hipSuccess)
+        return 0;
+    return cus * (per > 0 ? per : 1);
+}
+
+template <int I, int DIR, int LX>
+hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint32_t batch,
+                          bool stream, hipStream_t s) {
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
-    const dim3 grid((unsigned)((batch + B - 1) / B));
-    if (sign < 0)
-        hipLaunchKernelGGL((k_c2c_local<I, -1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
-    else
-        hipLaunchKernelGGL((k_c2c_local<I, 1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
+    const uint64_t groups = (batch + B - 1) / B;
+    if (stream) {
+        static int cap = 0;
+        if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream<I, DIR, LX>));
+        if (cap <= 0) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(groups < (uint64_t)cap ? groups : (uint64_t)cap));
+        hipLaunchKernelGGL((k_c2c_stream<I, DIR, LX>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
+    } else {
+        hipLaunchKernelGGL((k_c2c_local<I, DIR, LX>), dim3((unsigned)groups), dim3(bfly_plan::kBlock), 0, s,
+                           in, out, tw, batch);
+    }
+    return hipGetLastError();
+}
+
+template <int I, int DIR>
+hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw2g, uint32_t batch,
+                         hipStream_t s) {
+    constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
+    const uint64_t groups = (batch + B - 1) / B;
+    static int cap = 0;
+    if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>));
+    if (cap <= 0) return hipErrorInvalidValue;
+    const dim3 grid((unsigned)(groups < (uint64_t)cap ? groups : (uint64_t)cap));
+    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw2g, batch);
     return hipGetLastError();
 }
 
 template <int I>
+hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uint32_t batch,
+                            int sign, int variant, hipStream_t s) {
+    if (variant == BFLY_VARIANT_STREAM2) {
+        constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
+        const uint64_t groups = (batch + B - 1) / B;
+        const void* f = sign < 0 ? reinterpret_cast<const void*>(&k_c2c_stream2<I, -1>)
+                                 : reinterpret_cast<const void*>(&k_c2c_stream2<I, 1>);
+        static int cap[2] = {0, 0};
+        int& c = cap[sign < 0 ? 0 : 1];
+        if (c == 0) c = resident_blocks(f);
+        if (c <= 0) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
+        if (sign < 0)
+            hipLaunchKernelGGL((k_c2c_stream2<I, -1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
+        else
+            hipLaunchKernelGGL((k_c2c_stream2<I, 1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
+        return hipGetLastError();
+    }
+    if (variant == BFLY_VARIANT_STREAM_LT) {
+        const float2* tw2g = tw + bfly_host::table_layout(0, PlanOf<I>::N).extra;
+        return sign < 0 ? launch_c2c_lt<I, -1>(in, out, tw2g, batch, s)
+                        : launch_c2c_lt<I, 1>(in, out, tw2g, batch, s);
+    }
+    const bool stream = variant == BFLY_VARIANT_STREAM || variant == BFLY_VARIANT_STREAM_SWZ;
+    const bool swz = variant == BFLY_VARIANT_PLAIN_SWZ || variant == BFLY_VARIANT_STREAM_SWZ;
+    if (sign < 0)
+        return swz ? launch_c2c_dir<I, -1, LX_SWZ>(in, out, tw, batch, stream, s)
+                   : launch_c2c_dir<I, -1, LX_PAD>(in, out, tw, batch, stream, s);
+    return swz ? launch_c2c_dir<I, 1, LX_SWZ>(in, out, tw, batch, stream, s)
+               : launch_c2c_dir<I, 1, LX_PAD>(in, out, tw, batch, stream, s);
+}
+
+template <int I>
 hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, const float2* wn,
-                       uint32_t batch, hipStream_t s) {
+                       uint32_t batch, uint32_t bins, uint32_t stride, hipStream_t s) {
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
     const dim3 grid((unsigned)((batch + B - 1) / B));
     if (kind == 1)
-        hipLaunchKernelGGL((k_r2c<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch);
+        hipLaunchKernelGGL((k_r2c<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch,
+                           bins, stride);
     else
         hipLaunchKernelGGL((k_c2r<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch,
-                           1.0f / (float)(2 * PlanOf<I>::N));
+                           1.0f / (float)(2 * PlanOf<I>::N), bins, stride);
     return hipGetLastError();
 }
 
@@ -337,21 +596,21 @@ constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
 
 namespace bfly_kern {
 
-hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s) {
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     const float2* tw = reinterpret_cast<const float2*>(table);
     switch (n) {
-        case 16: return launch_c2c_local<idx(16)>(i2, o2, tw, batch, sign, s);
-        case 32: return launch_c2c_local<idx(32)>(i2, o2, tw, batch, sign, s);
-        case 64: return launch_c2c_local<idx(64)>(i2, o2, tw, batch, sign, s);
-        case 128: return launch_c2c_local<idx(128)>(i2, o2, tw, batch, sign, s);
-        case 256: return launch_c2c_local<idx(256)>(i2, o2, tw, batch, sign, s);
-        case 512: return launch_c2c_local<idx(512)>(i2, o2, tw, batch, sign, s);
-        case 1024: return launch_c2c_local<idx(1024)>(i2, o2, tw, batch, sign, s);
-        case 2048: return launch_c2c_local<idx(2048)>(i2, o2, tw, batch, sign, s);
-        case 4096: return launch_c2c_local<idx(4096)>(i2, o2, tw, batch, sign, s);
+        case 16: return launch_c2c_local<idx(16)>(i2, o2, tw, batch, sign, variant, s);
+        case 32: return launch_c2c_local<idx(32)>(i2, o2, tw, batch, sign, variant, s);
+        case 64: return launch_c2c_local<idx(64)>(i2, o2, tw, batch, sign, variant, s);
+        case 128: return launch_c2c_local<idx(128)>(i2, o2, tw, batch, sign, variant, s);
+        case 256: return launch_c2c_local<idx(256)>(i2, o2, tw, batch, sign, variant, s);
+        case 512: return launch_c2c_local<idx(512)>(i2, o2, tw, batch, sign, variant, s);
+        case 1024: return launch_c2c_local<idx(1024)>(i2, o2, tw, batch, sign, variant, s);
+        case 2048: return launch_c2c_local<idx(2048)>(i2, o2, tw, batch, sign, variant, s);
+        case 4096: return launch_c2c_local<idx(4096)>(i2, o2, tw, batch, sign, variant, s);
         default: break;
     }
     const bfly_host::TableLayout L = bfly_host::table_layout(0, n);
@@ -366,21 +625,21 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
     }
 }
 
-hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
-                const float* table, hipStream_t s) {
+hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                uint32_t stride, const float* table, hipStream_t s) {
     const float2* tw = reinterpret_cast<const float2*>(table);
     const bfly_host::TableLayout L = bfly_host::table_layout(kind, n);
     const float2* wn = tw + L.extra;
     switch (n / 2) {
-        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, s);
-        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, s);
-        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, s);
-        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, s);
-        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, s);
-        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, s);
-        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, s);
-        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, s);
-        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, s);
+        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, bins, stride, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -6,10 +6,10 @@
 
 namespace bfly_kern {
 
-hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s);
-hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
-                const float* table, hipStream_t s);
+hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                uint32_t stride, const float* table, hipStream_t s);
 hipError_t copy(const void* src, void* dst, size_t bytes, hipStream_t s);
 
 }  // namespace bfly_kern

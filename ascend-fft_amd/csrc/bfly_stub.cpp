@@ -92,12 +92,32 @@ int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
     if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return e;
     return launch();
 }
+int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
+                const float* table, void* work, void* stream) {
+    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) {
+        if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return e;
+        return BFLY_ERR_SHAPE;
+    }
+    return bfly_c2c(in, out, n, batch, sign, table, work, stream);
+}
 int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
     if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;
     return launch();
 }
 int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
     if (int e = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return e;
+    return launch();
+}
+int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                uint32_t stride, const float* table, void*) {
+    if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;
+    if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
+    return launch();
+}
+int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                uint32_t stride, const float* table, void*) {
+    if (int e = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return e;
+    if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
     return launch();
 }
 int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,

@@ -24,4 +24,10 @@ int fill_table(int kind, uint32_t n, float* dst);
 int check_execute(int kind, int sign, const void* in, const void* out, uint32_t n,
                   uint32_t batch, const void* table, const void* work);
 
+// Mode-truncated real transforms: 1 <= bins <= n/2+1, stride (floats) even
+// and at least 2*bins.
+bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride);
+// The variant BFLY_VARIANT_AUTO resolves to for a C2C of (n, batch).
+int auto_variant(uint32_t n, uint32_t batch);
+
 }  // namespace bfly_host

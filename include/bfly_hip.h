@@ -103,6 +103,33 @@ BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
 BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream);
 
+/* Kernel variants of one C2C length (same table, same results):
+ *   BFLY_VARIANT_AUTO    the library's choice for (n, batch)
+ *   BFLY_VARIANT_PLAIN   one workgroup per group of transforms, one pass over
+ *                        the batch (grid = batch / transforms-per-workgroup)
+ *   BFLY_VARIANT_STREAM  grid sized to the resident capacity; each workgroup
+ *                        walks the batch with register-resident twiddles and
+ *                        prefetches its next transform during the current one
+ *   BFLY_VARIANT_PLAIN_SWZ / BFLY_VARIANT_STREAM_SWZ  the same two with an
+ *                        XOR-swizzled, unpadded LDS exchange (32 KiB per
+ *                        256-thread workgroup instead of 34 KiB)
+ *   BFLY_VARIANT_STREAM_LT  streaming form with two 64-entry twiddle tables in
+ *                        LDS (W = A[m mod 64] * B[m / 64]) instead of
+ *                        register-resident twiddles
+ *   BFLY_VARIANT_STREAM2 streaming form with two transforms in flight per
+ *                        workgroup (swizzled LDS, register twiddles)
+ * Long lengths (n > 4096) have one variant; every id is accepted for them. */
+#define BFLY_VARIANT_AUTO 0
+#define BFLY_VARIANT_PLAIN 1
+#define BFLY_VARIANT_STREAM 2
+#define BFLY_VARIANT_PLAIN_SWZ 3
+#define BFLY_VARIANT_STREAM_SWZ 4
+#define BFLY_VARIANT_STREAM_LT 5
+#define BFLY_VARIANT_STREAM2 6
+#define BFLY_VARIANT_LAST 6
+BFLY_API int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+                         int variant, const float* table, void* work, void* stream);
+
 /* bfly_r2c replaces kfft_fwd(n/2) + kfft_r2c_post + the host repack of
  * Plan::runR2C (butterfly.cpp:753-806); one fused kernel, output written
  * directly at row stride n+2. */
@@ -115,6 +142,17 @@ BFLY_API int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, c
  * reference's Hermitian expansion does (src/framework/reference.cpp:64-79). */
 BFLY_API int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
              void* stream);
+
+/* Mode-truncated real transforms (the spectral-convolution layer keeps the
+ * first `bins` modes): bfly_r2c_ex writes bins 0..bins-1 of each row at a row
+ * stride of out_stride floats; bfly_c2r_ex reads bins 0..bins-1 at a row
+ * stride of in_stride floats and treats every higher bin as zero.
+ * 1 <= bins <= n/2+1; the stride is even and >= 2*bins.  bfly_r2c / bfly_c2r
+ * are these with bins = n/2+1 and stride = n+2. */
+BFLY_API int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                         uint32_t out_stride, const float* table, void* stream);
+BFLY_API int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                         uint32_t in_stride, const float* table, void* stream);
 
 /* ------------------------------------------------------------- probes */
 /* Device-to-device copy of `bytes` (16-byte aligned, multiple of 16) with

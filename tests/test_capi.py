@@ -68,17 +68,18 @@ PLANS = {16: [16], 32: [16, 2], 64: [16, 4], 128: [16, 8], 256: [16, 16], 512: [
 
 @pytest.mark.parametrize("n", sorted(PLANS))
 def test_c2c_table_is_fp64_twiddles(n):
+    # [pass tables][W_n^j, j < 64 | W_n^{64 j}, j < n/64], each entry the FP64
+    # value rounded once to fp32.
     lib = bfly.hip()
     nf = lib.bfly_table_floats(bfly.KIND_C2C, n)
     t = np.empty(nf, np.float32)
     assert lib.bfly_table_fill(bfly.KIND_C2C, n, t.ctypes.data) == 0
-    want = _stage_table(PLANS[n])
-    got = t[0::2] + 1j * t[1::2]
-    if len(PLANS[n]) == 1:
-        assert nf == 4 and not np.any(t)
-        return
-    assert got.size == want.size
-    # each entry is the FP64 value rounded once to fp32
+    stage = _stage_table(PLANS[n]) if len(PLANS[n]) > 1 else np.zeros(0, complex)
+    lo = min(n, 64)
+    two = np.concatenate([np.exp(-2j * np.pi * np.arange(lo) / n),
+                          np.exp(-2j * np.pi * ((lo * np.arange(n // lo)) % n) / n)])
+    want = np.concatenate([stage, two])
+    assert nf == 2 * want.size
     assert np.array_equal(t[0::2], want.real.astype(np.float32))
     assert np.array_equal(t[1::2], want.imag.astype(np.float32))
 

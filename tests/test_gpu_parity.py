@@ -58,6 +58,27 @@ def test_c2c_grid_seed1(n):
         gate_c2c(got, x, n)
 
 
+VARIANTS = [1, 2, 3, 4, 5, 6]  # plain, stream, plain+swizzle, stream+swizzle, stream+LDS twiddles, stream x2
+
+
+@pytest.mark.parametrize("variant", VARIANTS)
+@pytest.mark.parametrize("n", [16, 32, 64, 128, 256, 512, 1024, 2048, 4096])
+def test_c2c_every_variant(n, variant):
+    # Each kernel variant on ragged and multi-wave batches, both signs.
+    import torch
+    from bfly import _lib
+    for b, sign in ((1, -1), (97, 1), (3000, -1)):
+        x = oracle.uniform_input(13, (b, 2 * n))
+        xt = torch.from_numpy(x).cuda()
+        y = torch.empty_like(xt)
+        tab = _ops().table(0, n, xt.device)
+        rc = _lib.hip().bfly_c2c_ex(xt.data_ptr(), y.data_ptr(), n, b, sign, variant, tab.data_ptr(),
+                                    None, torch.cuda.current_stream().cuda_stream)
+        assert rc == 0
+        torch.cuda.synchronize()
+        gate_c2c(y.cpu().numpy(), x, n, sign)
+
+
 @pytest.mark.parametrize("n", [16, 32, 64, 1024, 4096])
 def test_c2c_inverse_is_conjugate_transform(n):
     b = 5
