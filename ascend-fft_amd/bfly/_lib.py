@@ -30,6 +30,8 @@ VARIANT_PLAIN = 1
 VARIANT_STREAM = 2
 VARIANT_PLAIN_SWZ = 3
 VARIANT_STREAM_SWZ = 4
+VARIANT_STREAM_LT = 5
+VARIANT_STREAM2 = 6
 
 KIND_C2C = 0
 KIND_R2C = 1

@@ -14,12 +14,21 @@ from . import _lib
 REL_L2_GATE = 1e-5
 
 
+VARIANT_NAMES = {
+    _lib.VARIANT_PLAIN: "plain",
+    _lib.VARIANT_STREAM: "stream",
+    _lib.VARIANT_PLAIN_SWZ: "plain_swizzle",
+    _lib.VARIANT_STREAM_SWZ: "stream_swizzle",
+    _lib.VARIANT_STREAM_LT: "stream_lds_twiddles",
+    _lib.VARIANT_STREAM2: "stream_depth2",
+}
+
+
 def variants(kind: int, n: int) -> list[int]:
-    """Variant ids the library implements for (kind, n); 0 is the plain path."""
-    lib = _lib.hip()
-    if not hasattr(lib, "bfly_variants"):
-        return [0]
-    return [0]
+    """C2C kernel variants of length n; the first is the plain one-shot grid."""
+    if kind != _lib.KIND_C2C or n > 4096:
+        return [_lib.VARIANT_PLAIN]
+    return list(VARIANT_NAMES)
 
 
 def rel_l2_fp64(got: np.ndarray, x: np.ndarray, n: int, sign: int = -1) -> float:
@@ -34,41 +43,53 @@ def rel_l2_fp64(got: np.ndarray, x: np.ndarray, n: int, sign: int = -1) -> float
     return float(np.max(num / den))
 
 
-def time_variant(x, y, n: int, batch: int, variant: int, reps: int = 20) -> float:
+def launch_variant(x, y, n: int, batch: int, variant: int, sign: int = -1) -> None:
     import torch
     from . import ops
-    for _ in range(3):
-        ops.c2c_raw(x, y, n, batch, -1)
+    tab = ops.table(_lib.KIND_C2C, n, x.device)
+    rc = _lib.hip().bfly_c2c_ex(x.data_ptr(), y.data_ptr(), n, batch, sign, variant, tab.data_ptr(),
+                                None, torch.cuda.current_stream(x.device).cuda_stream)
+    _lib.check(rc, f"bfly_c2c_ex(n={n}, batch={batch}, variant={variant})")
+
+
+def time_variant(xs, ys, n: int, batch: int, variant: int, reps: int = 30) -> float:
+    """Mean ms per launch over `reps` launches cycling through the buffer pairs."""
+    import torch
+    for i in range(3):
+        launch_variant(xs[i % len(xs)], ys[i % len(ys)], n, batch, variant)
     torch.cuda.synchronize()
     e0 = torch.cuda.Event(enable_timing=True)
     e1 = torch.cuda.Event(enable_timing=True)
     e0.record()
-    for _ in range(reps):
-        ops.c2c_raw(x, y, n, batch, -1)
+    for i in range(reps):
+        launch_variant(xs[i % len(xs)], ys[i % len(ys)], n, batch, variant)
     e1.record()
     torch.cuda.synchronize()
     return e0.elapsed_time(e1) / reps
 
 
-def chosen_vs_plain(n: int, batch: int, device, host_input: np.ndarray, sample_rows) -> dict:
-    """Pick the fastest gate-passing variant for C2C (n, batch); report its
-    speedup over the plain variant and its error vs FP64."""
+def chosen_vs_plain(n: int, batch: int, pairs, host_input: np.ndarray, sample_rows) -> dict:
+    """Time every C2C variant of (n, batch) on the same rotated buffer pairs,
+    gate each against FP64 on sample rows, and report the fastest passing one
+    next to the plain one-shot variant."""
     import torch
-    x = torch.from_numpy(host_input).to(device)
-    y = torch.empty_like(x)
+    xs = [p[0] for p in pairs]
+    ys = [p[1] for p in pairs]
+    idx = torch.as_tensor(sample_rows, device=xs[0].device)
     results = []
     for v in variants(_lib.KIND_C2C, n):
-        ms = time_variant(x, y, n, batch, v)
-        got = y[torch.as_tensor(sample_rows, device=device)].cpu().numpy()
+        ms = time_variant(xs, ys, n, batch, v)
+        launch_variant(xs[0], ys[0], n, batch, v)
+        got = ys[0][idx].cpu().numpy()
         rel = rel_l2_fp64(got, host_input[sample_rows], n)
         results.append({"variant": v, "ms": ms, "rel_l2": rel})
     plain = results[0]
     ok = [r for r in results if r["rel_l2"] <= REL_L2_GATE]
     best = min(ok, key=lambda r: r["ms"]) if ok else plain
     return {
-        "plan": "plain" if best["variant"] == 0 else f"variant{best['variant']}",
+        "plan": VARIANT_NAMES.get(best["variant"], str(best["variant"])),
         "speedup_vs_plain": round(plain["ms"] / best["ms"], 4),
         "rel_l2_vs_fp64": best["rel_l2"],
-        "candidates": [{"variant": r["variant"], "us": round(r["ms"] * 1e3, 2),
-                        "rel_l2": r["rel_l2"]} for r in results],
+        "candidates": [{"variant": VARIANT_NAMES.get(r["variant"], str(r["variant"])),
+                        "us": round(r["ms"] * 1e3, 2), "rel_l2": r["rel_l2"]} for r in results],
     }

@@ -76,17 +76,19 @@ __device__ __forceinline__ void fft_local(float2 (&v)[bfly_plan::kPoints], float
 }
 
 // ------------------------------------------------------------------ C2C
-template <int I, int DIR, int LX>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
-    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-    uint32_t batch) {
+// One work unit: the WG/T transforms g*B .. g*B+B-1 of a batch (B = WG/T),
+// one HBM read and one HBM write per element.  `lds` holds B regions.
+template <int I, int DIR, int LX, int WG>
+__device__ __forceinline__ void c2c_unit(const float2* __restrict__ in, float2* __restrict__ out,
+                                         const float2* __restrict__ tw, uint32_t batch, uint32_t g,
+                                         float2* lds) {
     using PL = PlanOf<I>;
-    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int N = PL::N, T = PL::T, B = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    __shared__ float2 lds[B * lx_len<LX>(N)];
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
     const int tid = threadIdx.x;
     const int bl = tid / T, t = tid % T;
-    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const uint64_t tr = (uint64_t)g * B + bl;
     const bool active = tr < batch;
     float2 v[P];
     if (active) {
@@ -108,6 +110,15 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
 #pragma unroll
             for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], v[m * RL + r]);
     }
+}
+
+template <int I, int DIR, int LX>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+    uint32_t batch) {
+    constexpr int N = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
+    __shared__ float2 lds[B * lx_len<LX>(N)];
+    c2c_unit<I, DIR, LX, bfly_plan::kBlock>(in, out, tw, batch, blockIdx.x, lds);
 }
 
 // C2C, streaming form: a grid sized to the resident-workgroup capacity walks
@@ -278,20 +289,18 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
 // Writes bins k < kout of each row at a row stride of ostride floats (the
 // full transform: kout = M+1, ostride = 2M+2; a mode-truncated one writes
 // only its first kout bins).
-template <int I>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restrict__ in,
-                                                           float* __restrict__ out,
-                                                           const float2* __restrict__ tw,
-                                                           const float2* __restrict__ wn,
-                                                           uint32_t batch, uint32_t kout,
-                                                           uint32_t ostride) {
+template <int I, int WG>
+__device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __restrict__ out,
+                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                         uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
+                                         float2* lds) {
     using PL = PlanOf<I>;
-    constexpr int M = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int M = PL::N, T = PL::T, B = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    __shared__ float2 lds[B * lds_len(M)];
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
     const int tid = threadIdx.x;
     const int bl = tid / T, t = tid % T;
-    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const uint64_t tr = (uint64_t)g * B + bl;
     const bool active = tr < batch;
     float2* L = lds + bl * lds_len(M);
     float2 v[P];
@@ -313,22 +322,38 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
 #pragma unroll
         for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
     __syncthreads();
-    if (!active) return;
-    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
+    if (active) {
+        float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
 #pragma unroll
-    for (int i = 0; i <= P; i++) {
-        const int k = t + i * T;
-        if (k <= M && k < (int)kout) {
-            const float2 a = L[lds_idx(k & (M - 1))];
-            const float2 b = L[lds_idx((M - k) & (M - 1))];
-            const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
-            // O = (a - conj b) / (2i) = (0.5 (a.y + b.y), -0.5 (a.x - b.x))
-            const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
-            const float2 w = wn[k];
-            const float2 wo = cmul_tw<-1>(o, w);
-            st_io<(T >= 16)>(&dst[k], cadd(e, wo));
+        for (int i = 0; i <= P; i++) {
+            const int k = t + i * T;
+            if (k <= M && k < (int)kout) {
+                const float2 a = L[lds_idx(k & (M - 1))];
+                const float2 b = L[lds_idx((M - k) & (M - 1))];
+                const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+                // O = (a - conj b) / (2i) = (0.5 (a.y + b.y), -0.5 (a.x - b.x))
+                const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+                const float2 w = wn[k];
+                const float2 wo = cmul_tw<-1>(o, w);
+                st_io<(T >= 16)>(&dst[k], cadd(e, wo));
+            }
         }
     }
+}
+
+// z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
+//   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
+//   X[k] = E + W_n^k O            (Z[M] == Z[0]).
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restrict__ in,
+                                                           float* __restrict__ out,
+                                                           const float2* __restrict__ tw,
+                                                           const float2* __restrict__ wn,
+                                                           uint32_t batch, uint32_t kout,
+                                                           uint32_t ostride) {
+    constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
+    __shared__ float2 lds[B * lds_len(M)];
+    r2c_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, kout, ostride, blockIdx.x, lds);
 }
 
 // ------------------------------------------------------------------ C2R
@@ -336,21 +361,19 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
 // with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
 // Reads bins k < kin of each row at a row stride of istride floats; bins at
 // or above kin are zero (the full transform: kin = M+1, istride = 2M+2).
-template <int I>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restrict__ in,
-                                                           float* __restrict__ out,
-                                                           const float2* __restrict__ tw,
-                                                           const float2* __restrict__ wn,
-                                                           uint32_t batch, float scale, uint32_t kin,
-                                                           uint32_t istride) {
+template <int I, int WG>
+__device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __restrict__ out,
+                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                         uint32_t batch, float scale, uint32_t kin, uint32_t istride,
+                                         uint32_t g, float2* lds) {
     using PL = PlanOf<I>;
-    constexpr int M = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int M = PL::N, T = PL::T, B = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
     static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
-    __shared__ float2 lds[B * lds_len(M)];
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
     const int tid = threadIdx.x;
     const int bl = tid / T, t = tid % T;
-    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const uint64_t tr = (uint64_t)g * B + bl;
     const bool active = tr < batch;
     float2* L = lds + bl * lds_len(M);
     if (active) {
@@ -386,6 +409,18 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restri
             const float2 z = v[m * RL + r];
             st_io<(T >= 16)>(&dst[t + m * T + r * (M / RL)], make_float2(z.x * scale, z.y * scale));
         }
+}
+
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restrict__ in,
+                                                           float* __restrict__ out,
+                                                           const float2* __restrict__ tw,
+                                                           const float2* __restrict__ wn,
+                                                           uint32_t batch, float scale, uint32_t kin,
+                                                           uint32_t istride) {
+    constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
+    __shared__ float2 lds[B * lds_len(M)];
+    c2r_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, scale, kin, istride, blockIdx.x, lds);
 }
 
 // ------------------------------------------------------------- long C2C

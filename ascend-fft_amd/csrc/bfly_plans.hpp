@@ -18,6 +18,7 @@ constexpr int kBlock = 256;       // threads per workgroup for N <= 4096
 constexpr int kMaxRadices = 5;
 constexpr int kMinN = 16;
 constexpr int kMaxLocalN = 4096;  // largest length held by one 256-thread workgroup
+constexpr int kMaxSingleN = 16384;  // largest length held by one workgroup (1024 threads)
 
 struct Plan {
     int n;
@@ -35,6 +36,10 @@ constexpr Plan kPlans[] = {
     {1024, 3, {16, 16, 4, 0, 0}},
     {2048, 3, {16, 16, 8, 0, 0}},
     {4096, 3, {16, 16, 16, 0, 0}},
+    // Single-workgroup plans of the long lengths (T = n/16 = 512 / 1024
+    // threads, the whole transform LDS-resident: 68 / 136 KiB).
+    {8192, 4, {16, 16, 16, 2, 0}},
+    {16384, 4, {16, 16, 16, 4, 0}},
 };
 constexpr int kNumPlans = (int)(sizeof(kPlans) / sizeof(kPlans[0]));
 

@@ -73,6 +73,10 @@ TableLayout table_layout(int kind, uint32_t n) {
         t.stage1 = bfly_plan::stage_twiddle_len(bfly_plan::plan_index(n1));
         t.extra = t.stage1 + bfly_plan::stage_twiddle_len(bfly_plan::plan_index(n2));
         t.complex_len = t.extra + n;  // W_N^m, m < N
+        if (bfly_plan::plan_index(n) >= 0) {
+            t.single = t.complex_len;
+            t.complex_len += bfly_plan::stage_twiddle_len(bfly_plan::plan_index(n));
+        }
     } else {
         const uint32_t m = n / 2;
         t.stage0 = 0;
@@ -97,6 +101,7 @@ int fill_table(int kind, uint32_t n, float* dst) {
         fill_stage(bfly_plan::plan_index(n1), dst + 2 * t.stage0);
         fill_stage(bfly_plan::plan_index(n2), dst + 2 * t.stage1);
         for (uint64_t m = 0; m < n; m++) twiddle(m, n, dst + 2 * (t.extra + m));
+        if (t.single) fill_stage(bfly_plan::plan_index(n), dst + 2 * t.single);
     } else {
         const uint32_t m = n / 2;
         fill_stage(bfly_plan::plan_index(m), dst);

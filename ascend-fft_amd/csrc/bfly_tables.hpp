@@ -10,10 +10,13 @@ namespace bfly_host {
 //   local C2C:  [stage table of n]
 //   long C2C:   [stage table of N1][stage table of N2][W_N^m, m < N]
 //   R2C / C2R:  [stage table of n/2][W_n^k, k = 0..n/2, padded to even]
+//   long C2C with a single-workgroup plan (n <= 16384) appends
+//               [stage table of the single-workgroup plan of n]
 struct TableLayout {
     size_t stage0 = 0;      // first stage table
     size_t stage1 = 0;      // second stage table (long C2C only)
     size_t extra = 0;       // W_N (long) or W_n (real) table
+    size_t single = 0;      // single-workgroup pass table (long C2C, n <= 16384)
     size_t complex_len = 0; // total complex entries (0 = unsupported)
 };
 

@@ -1,16 +1,17 @@
 #!/usr/bin/env python3
 """Headline benchmark: plain fp32 C2C forward, N=4096, batch=4096, on one MI355X.
 
-One step = one bfly_c2c launch over the whole batch ([4096][8192] float32,
-interleaved), inputs resident in HBM.  Prints ONE JSON line:
+One step = one bfly_c2c launch (the library's automatic variant choice) over
+the whole batch ([4096][8192] float32, interleaved), inputs resident in HBM.  Prints ONE JSON line:
   value          algorithmic GB/s = 16*N*batch bytes / mean step time
   roofline       achieved GB/s of the kernel (HIP events on its stream) vs the
                  8 TB/s HBM peak, plus HBM traffic per launch from the committed
                  rocprofv3 counter summary (profiles/), or null
   cpu_baseline   the reference's own FP64 CPU FFT (oracle/_ref, built from the
                  reference sources) timed on this host on a bounded sample
-  chosen         the plan the measured selection picks for this shape, its
-                 speedup over the plain plan and its relative error vs FP64
+  chosen         every kernel variant of this shape timed on the same buffers
+                 and gated against FP64; the fastest passing one, its speedup
+                 over the plain one-shot variant and its relative error
 
 Timed steps rotate over several input/output buffer pairs (>= 1 GiB in
 total) so that the 256 MB last-level cache cannot serve one step's input from
@@ -142,7 +143,7 @@ def main():
     got = y0[sample].cpu().numpy()
     rel = tune.rel_l2_fp64(got, host[sample], N)
 
-    chosen = tune.chosen_vs_plain(N, BATCH, dev, host, sample)
+    chosen = tune.chosen_vs_plain(N, BATCH, pairs, host, sample)
 
     if args.probe_copy:
         # Known-size traffic for calibrating FETCH_SIZE / WRITE_SIZE:

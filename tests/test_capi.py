@@ -96,7 +96,12 @@ def test_long_table_layout(n):
     assert np.array_equal(c[:s1.size], s1.astype(np.complex64))
     assert np.array_equal(c[s1.size:s1.size + s2.size], s2.astype(np.complex64))
     w = np.exp(-2j * np.pi * np.arange(n) / n).astype(np.complex64)
-    assert np.array_equal(c[s1.size + s2.size:], w)
+    o = s1.size + s2.size
+    assert np.array_equal(c[o:o + n], w)
+    # lengths a single workgroup holds append that plan's inter-pass table
+    single = {8192: [16, 16, 16, 2], 16384: [16, 16, 16, 4]}.get(n)
+    tail = _stage_table(single).astype(np.complex64) if single else np.zeros(0, np.complex64)
+    assert np.array_equal(c[o + n:], tail)
     assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 64) == n * 64 * 8
 
 
@@ -145,3 +150,17 @@ def test_execute_guards_through_stub():
     cnt = (C.c_int * 8)()
     s.bfly_stub_counters(cnt)
     assert cnt[3] == 3  # launches counted only for accepted calls
+
+
+def test_python_modules_import_and_variant_ids_match_header():
+    import importlib
+    for m in ("bfly.ops", "bfly.tune", "bfly.autograd", "bfly.spectral"):
+        importlib.import_module(m)
+    text = HEADER.read_text()
+    for name in ("AUTO", "PLAIN", "STREAM", "PLAIN_SWZ", "STREAM_SWZ", "STREAM_LT", "STREAM2"):
+        m = re.search(rf"#define BFLY_VARIANT_{name} (\d+)", text)
+        assert m and int(m.group(1)) == getattr(_lib, f"VARIANT_{name}"), name
+    for name in ("OK", "ERR_NOT_LOADED", "ERR_TABLE", "ERR_ALLOC", "ERR_H2D", "ERR_LAUNCH",
+                 "ERR_SYNC", "ERR_D2H", "ERR_SHAPE"):
+        m = re.search(rf"#define BFLY_{name} \(?(-?\d+)\)?", text)
+        assert m and int(m.group(1)) == getattr(_lib, name), name
