@@ -32,6 +32,9 @@ VARIANT_PLAIN_SWZ = 3
 VARIANT_STREAM_SWZ = 4
 VARIANT_STREAM_LT = 5
 VARIANT_STREAM2 = 6
+VARIANT_FOUR_STEP = 7
+VARIANT_SPLIT_MFMA = 8
+VARIANT_LAST = 8
 
 KIND_C2C = 0
 KIND_R2C = 1
@@ -54,9 +57,21 @@ SYMBOLS = (
     "bfly_rt_malloc", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
     "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
-    "bfly_c2r_ex", "bfly_probe_copy",
-    "bfly_probe_device",
+    "bfly_c2r_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
+    "bfly_group_query", "bfly_group_destroy", "bfly_probe_copy", "bfly_probe_device",
 )
+
+
+class GroupRequest(C.Structure):
+    """bfly_group_request of include/bfly_hip.h."""
+    _fields_ = [("in_", C.c_void_p), ("out", C.c_void_p), ("kind", C.c_int32),
+                ("sign", C.c_int32), ("n", C.c_uint32), ("batch", C.c_uint32)]
+
+
+class GroupConfig(C.Structure):
+    """bfly_group_config of include/bfly_hip.h."""
+    _fields_ = [("workgroup", C.c_int32), ("schedule", C.c_int32), ("order", C.c_int32),
+                ("dynamic", C.c_int32), ("grid", C.c_int32)]
 
 
 class BflyError(RuntimeError):
@@ -88,6 +103,12 @@ def _bind(lib):
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_r2c_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
         "bfly_c2r_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
+        "bfly_group_supported": (i32, [i32, u32, i32]),
+        "bfly_group_create": (i32, [C.POINTER(GroupRequest), i32, C.POINTER(GroupConfig),
+                                    C.POINTER(vp)]),
+        "bfly_group_launch": (i32, [vp, vp, vp]),
+        "bfly_group_query": (i32, [vp, C.POINTER(u32), C.POINTER(u32), C.POINTER(sz)]),
+        "bfly_group_destroy": (i32, [vp]),
         "bfly_probe_copy": (i32, [vp, vp, sz, vp]),
         "bfly_probe_device": (i32, [i32, C.c_char_p, sz]),
     }

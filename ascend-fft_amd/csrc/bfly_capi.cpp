@@ -76,7 +76,8 @@ int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
 
 int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                 const float* table, void* work, void* stream) {
-    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return g;
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work, variant))
+        return g;
     if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
     if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_variant(n, batch);
     return launched(bfly_kern::c2c(in, out, n, batch, sign, variant, table, work, S(stream)));
@@ -117,20 +118,22 @@ int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint
     if (!ms || reps <= 0) return BFLY_ERR_SHAPE;
     hipEvent_t e0, e1;
     if (hipEventCreate(&e0) != hipSuccess) return BFLY_ERR_ALLOC;
-    if (hipEventCreate(&e1) != hipSuccess) { hipEventDestroy(e0); return BFLY_ERR_ALLOC; }
-    int rc = BFLY_OK;
-    hipEventRecord(e0, S(stream));
+    if (hipEventCreate(&e1) != hipSuccess) {
+        (void)hipEventDestroy(e0);
+        return BFLY_ERR_ALLOC;
+    }
+    int rc = hipEventRecord(e0, S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH;
     for (int i = 0; i < reps && rc == BFLY_OK; i++) {
         if (kind == BFLY_KIND_C2C) rc = bfly_c2c(in, out, n, batch, sign, table, work, stream);
         else if (kind == BFLY_KIND_R2C) rc = bfly_r2c(in, out, n, batch, table, stream);
         else if (kind == BFLY_KIND_C2R) rc = bfly_c2r(in, out, n, batch, table, stream);
         else rc = BFLY_ERR_SHAPE;
     }
-    hipEventRecord(e1, S(stream));
+    if (hipEventRecord(e1, S(stream)) != hipSuccess && rc == BFLY_OK) rc = BFLY_ERR_LAUNCH;
     if (hipEventSynchronize(e1) != hipSuccess && rc == BFLY_OK) rc = BFLY_ERR_SYNC;
     if (rc == BFLY_OK && hipEventElapsedTime(ms, e0, e1) != hipSuccess) rc = BFLY_ERR_SYNC;
-    hipEventDestroy(e0);
-    hipEventDestroy(e1);
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
     return rc;
 }
 
@@ -144,20 +147,28 @@ int bfly_probe_device(int device, char* json, size_t cap) {
     if (!json || cap == 0) return BFLY_ERR_SHAPE;
     hipDeviceProp_t p;
     if (hipGetDeviceProperties(&p, device) != hipSuccess) return BFLY_ERR_NOT_LOADED;
+    // an attribute the runtime cannot report stays 0 in the JSON
     int lds_per_cu = 0, mem_clock_khz = 0, bus_width = 0, l2 = 0;
-    hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device);
-    hipDeviceGetAttribute(&mem_clock_khz, hipDeviceAttributeMemoryClockRate, device);
-    hipDeviceGetAttribute(&bus_width, hipDeviceAttributeMemoryBusWidth, device);
-    hipDeviceGetAttribute(&l2, hipDeviceAttributeL2CacheSize, device);
+    if (hipDeviceGetAttribute(&lds_per_cu, hipDeviceAttributeMaxSharedMemoryPerMultiprocessor, device) != hipSuccess)
+        lds_per_cu = 0;
+    if (hipDeviceGetAttribute(&mem_clock_khz, hipDeviceAttributeMemoryClockRate, device) != hipSuccess)
+        mem_clock_khz = 0;
+    if (hipDeviceGetAttribute(&bus_width, hipDeviceAttributeMemoryBusWidth, device) != hipSuccess)
+        bus_width = 0;
+    if (hipDeviceGetAttribute(&l2, hipDeviceAttributeL2CacheSize, device) != hipSuccess) l2 = 0;
+    int wall_khz = 0;
+    if (hipDeviceGetAttribute(&wall_khz, hipDeviceAttributeWallClockRate, device) != hipSuccess)
+        wall_khz = 0;
     const int w = std::snprintf(
         json, cap,
         "{\"name\": \"%s\", \"gcn_arch\": \"%s\", \"compute_units\": %d, \"wave_size\": %d, "
         "\"lds_bytes_per_workgroup\": %zu, \"lds_bytes_per_cu\": %d, \"l2_bytes\": %d, "
         "\"global_mem_total_bytes\": %zu, \"max_threads_per_block\": %d, "
-        "\"clock_khz\": %d, \"mem_clock_khz\": %d, \"mem_bus_width_bits\": %d}",
+        "\"clock_khz\": %d, \"mem_clock_khz\": %d, \"mem_bus_width_bits\": %d, "
+        "\"wall_clock_khz\": %d}",
         p.name, p.gcnArchName, p.multiProcessorCount, p.warpSize, (size_t)p.sharedMemPerBlock,
         lds_per_cu, l2, (size_t)p.totalGlobalMem, p.maxThreadsPerBlock, p.clockRate,
-        mem_clock_khz, bus_width);
+        mem_clock_khz, bus_width, wall_khz);
     return (w > 0 && (size_t)w < cap) ? BFLY_OK : BFLY_ERR_SHAPE;
 }
 

@@ -199,21 +199,25 @@ template <int LX>
 __host__ __device__ constexpr int lx_len(int n) { return LX == LX_SWZ ? n : lds_len(n); }
 
 // Address of element r of a Stockham write group (base + r*Ns) and of a read
-// group (u + r*S).  In the padded layout every offset r*Ns (Ns >= 16) or
-// r*S (S >= 16) is a multiple of 16, so pad(x + c) = pad(x) + c*17/16 and
-// the whole group is one per-thread base plus compile-time offsets (LDS
+// group (u + r*S).  In the padded layout, when every offset r*Ns (Ns >= 16)
+// or r*S (S >= 16) is a multiple of 16, pad(x + c) = pad(x) + c*17/16 and the
+// whole group is one per-thread base plus compile-time offsets (LDS
 // immediates); for Ns == 1 the group never crosses a 16-slot boundary
-// (R | 16), so pad(base + r) = pad(base) + r.
+// (R | 16), so pad(base + r) = pad(base) + r.  Other strides (a small radix
+// ahead of radix 16) pad every address on its own.
+template <int Q>
+constexpr bool pad_aligned() { return Q % (1 << PAD_SHIFT) == 0; }
 template <int LX, int Ns>
 __device__ __forceinline__ int lx_wbase(int rb, int base) {
-    if constexpr (LX == LX_PAD) return lds_idx(base);
+    if constexpr (LX == LX_PAD) return (Ns == 1 || pad_aligned<Ns>()) ? lds_idx(base) : base;
     else return rb + base;
 }
 template <int LX, int Ns>
 __device__ __forceinline__ int lx_wat(int wb, int r) {
     if constexpr (LX == LX_PAD) {
-        static_assert(Ns == 1 || Ns % 16 == 0, "pass offsets must be multiples of the pad period");
-        return Ns == 1 ? wb + r : wb + r * Ns + (r * Ns >> PAD_SHIFT);
+        if constexpr (Ns == 1) return wb + r;
+        else if constexpr (pad_aligned<Ns>()) return wb + r * Ns + (r * Ns >> PAD_SHIFT);
+        else return lds_idx(wb + r * Ns);
     } else {
         const int g = wb + r * Ns;
         return g ^ ((g >> 5) & 31);
@@ -222,16 +226,16 @@ __device__ __forceinline__ int lx_wat(int wb, int r) {
 template <int LX, int S>
 __device__ __forceinline__ int lx_rat(int rbase, int r) {
     if constexpr (LX == LX_PAD) {
-        static_assert(S % 16 == 0, "read stride must be a multiple of the pad period");
-        return rbase + r * S + (r * S >> PAD_SHIFT);
+        if constexpr (pad_aligned<S>()) return rbase + r * S + (r * S >> PAD_SHIFT);
+        else return lds_idx(rbase + r * S);
     } else {
         const int g = rbase + r * S;
         return g ^ ((g >> 5) & 31);
     }
 }
-template <int LX>
+template <int LX, int S>
 __device__ __forceinline__ int lx_rbase(int rb, int u) {
-    if constexpr (LX == LX_PAD) return lds_idx(u);
+    if constexpr (LX == LX_PAD) return pad_aligned<S>() ? lds_idx(u) : u;
     else return rb + u;
 }
 
@@ -301,7 +305,7 @@ __device__ __forceinline__ void stockham(float2 (&v)[P], float2* lds, int rb,
         constexpr int M2 = P / R2;
 #pragma unroll
         for (int m = 0; m < M2; m++) {
-            const int rbase = lx_rbase<LX>(rb, t + m * T);
+            const int rbase = lx_rbase<LX, N / R2>(rb, t + m * T);
 #pragma unroll
             for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
         }
@@ -370,7 +374,7 @@ __device__ __forceinline__ void stockham_rt(float2 (&v)[P], float2* lds, int rb,
         constexpr int M2 = P / R2;
 #pragma unroll
         for (int m = 0; m < M2; m++) {
-            const int rbase = lx_rbase<LX>(rb, t + m * T);
+            const int rbase = lx_rbase<LX, N / R2>(rb, t + m * T);
 #pragma unroll
             for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
         }
@@ -425,7 +429,7 @@ __device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb,
         constexpr int M2 = P / R2;
 #pragma unroll
         for (int m = 0; m < M2; m++) {
-            const int rbase = lx_rbase<LX>(rb, t + m * T);
+            const int rbase = lx_rbase<LX, N / R2>(rb, t + m * T);
 #pragma unroll
             for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
         }

@@ -2,6 +2,9 @@
 //
 //   k_c2c_local  C2C, n <= 4096: one HBM read + one HBM write per element;
 //                256/T transforms per 256-thread workgroup (T = n/16).
+//   k_c2c_stream* persistent streaming forms of the same transform.
+//   k_c2c_single C2C, n = 8192 / 16384: one transform per 512 / 1024-thread
+//                workgroup, still one HBM round trip.
 //   k_r2c        R2C as an n/2-point complex FFT with the split post-twiddle
 //                fused behind it; writes the n/2+1 bins at row stride n+2.
 //   k_c2r        C2R: Hermitian pre-twiddle fused in front of an n/2-point
@@ -20,105 +23,32 @@
 #include "bfly_kernels.hpp"
 #include "bfly_plans.hpp"
 #include "bfly_tables.hpp"
+#include "bfly_units.hpp"
 
 using namespace bfly_dev;
 using bfly_plan::kPlans;
 
 namespace {
 
-template <int I>
-struct PlanOf {
-    static constexpr int N = kPlans[I].n;
-    static constexpr int NRAD = kPlans[I].nrad;
-    static constexpr int R0 = kPlans[I].rad[0];
-    static constexpr int RL = kPlans[I].rad[NRAD - 1];
-    static constexpr int P = bfly_plan::kPoints;
-    static constexpr int T = N / P;  // threads per transform
-};
-
-template <int I, int T, int DIR, int LX, size_t... S>
-__device__ __forceinline__ void run_plan(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
-                                         const float2* __restrict__ tw, int t,
-                                         std::index_sequence<S...>) {
-    stockham<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tw, t);
-}
-
-template <int I, int T, int DIR, int LX, size_t... S>
-__device__ __forceinline__ void run_plan_rt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
-                                            const float2* tr, int t, std::index_sequence<S...>) {
-    stockham_rt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tr, t);
-}
-
-template <int I, int T, int DIR, int LX, size_t... S>
-__device__ __forceinline__ void run_plan_lt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
-                                            const float2* tlo, const float2* thi, int t,
-                                            std::index_sequence<S...>) {
-    stockham_lt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tlo, thi, t);
-}
-
-template <int I, int T, size_t... S>
-__device__ __forceinline__ void load_plan_twiddles(float2* tr, const float2* __restrict__ tw, int t,
-                                                   std::index_sequence<S...>) {
-    load_twiddles<kPlans[I].n, bfly_plan::kPoints, T, 1, kPlans[I].rad[S]...>(tr, tw, t);
-}
-
-template <int I, size_t... S>
-constexpr int plan_twiddle_regs(std::index_sequence<S...>) {
-    return TwRegCount<bfly_plan::kPoints, 1, kPlans[I].rad[S]...>::value;
-}
-
-// Padded-region form used by the real and long kernels: `lds` is the
-// transform's own region.
-template <int I, int T, int DIR>
-__device__ __forceinline__ void fft_local(float2 (&v)[bfly_plan::kPoints], float2* lds,
-                                          const float2* __restrict__ tw, int t) {
-    run_plan<I, T, DIR, LX_PAD>(v, lds, 0, tw, t, std::make_index_sequence<kPlans[I].nrad>{});
-}
-
 // ------------------------------------------------------------------ C2C
-// One work unit: the WG/T transforms g*B .. g*B+B-1 of a batch (B = WG/T),
-// one HBM read and one HBM write per element.  `lds` holds B regions.
-template <int I, int DIR, int LX, int WG>
-__device__ __forceinline__ void c2c_unit(const float2* __restrict__ in, float2* __restrict__ out,
-                                         const float2* __restrict__ tw, uint32_t batch, uint32_t g,
-                                         float2* lds) {
-    using PL = PlanOf<I>;
-    constexpr int N = PL::N, T = PL::T, B = WG / T;
-    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    static_assert(WG % T == 0, "a work unit holds whole transforms");
-    const int tid = threadIdx.x;
-    const int bl = tid / T, t = tid % T;
-    const uint64_t tr = (uint64_t)g * B + bl;
-    const bool active = tr < batch;
-    float2 v[P];
-    if (active) {
-        const float2* src = in + tr * N;
-#pragma unroll
-        for (int m = 0; m < P / R0; m++)
-#pragma unroll
-            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
-    }
-    run_plan<I, T, DIR, LX>(v, LX == LX_SWZ ? lds : lds + bl * lds_len(N), LX == LX_SWZ ? bl * N : 0,
-                            tw, t, std::make_index_sequence<PL::NRAD>{});
-    if (active) {
-        float2* dst = out + tr * N;
-#pragma unroll
-        for (int m = 0; m < P / RL; m++)
-#pragma unroll
-            for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], v[m * RL + r]);
-    }
-}
-
 template <int I, int DIR, int LX>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_local(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
     uint32_t batch) {
     constexpr int N = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
     __shared__ float2 lds[B * lx_len<LX>(N)];
-    c2c_unit<I, DIR, LX, bfly_plan::kBlock>(in, out, tw, batch, blockIdx.x, lds);
+    c2c_unit<I, DIR, LX, bfly_plan::kBlock>(in, out, tw, batch, blockIdx.x, lds, threadIdx.x);
+}
+
+// C2C of a length one 256-thread workgroup cannot hold (n = 8192, 16384):
+// one transform per workgroup of T = n/16 threads (512 / 1024), the whole
+// transform LDS-resident (68 / 136 KiB), still one HBM round trip.
+template <int I, int DIR>
+__global__ __launch_bounds__(PlanOf<I>::T) void k_c2c_single(
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+    uint32_t batch) {
+    __shared__ float2 lds[lds_len(PlanOf<I>::N)];
+    c2c_unit<I, DIR, LX_PAD, PlanOf<I>::T>(in, out, tw, batch, blockIdx.x, lds, threadIdx.x);
 }
 
 // C2C, streaming form: a grid sized to the resident-workgroup capacity walks
@@ -286,64 +216,6 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
 // z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
 //   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
 //   X[k] = E + W_n^k O            (Z[M] == Z[0]).
-// Writes bins k < kout of each row at a row stride of ostride floats (the
-// full transform: kout = M+1, ostride = 2M+2; a mode-truncated one writes
-// only its first kout bins).
-template <int I, int WG>
-__device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __restrict__ out,
-                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
-                                         uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
-                                         float2* lds) {
-    using PL = PlanOf<I>;
-    constexpr int M = PL::N, T = PL::T, B = WG / T;
-    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    static_assert(WG % T == 0, "a work unit holds whole transforms");
-    const int tid = threadIdx.x;
-    const int bl = tid / T, t = tid % T;
-    const uint64_t tr = (uint64_t)g * B + bl;
-    const bool active = tr < batch;
-    float2* L = lds + bl * lds_len(M);
-    float2 v[P];
-    if (active) {
-        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)(2 * M));
-#pragma unroll
-        for (int m = 0; m < P / R0; m++)
-#pragma unroll
-            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (M / R0)]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
-    }
-    fft_local<I, T, -1>(v, L, tw, t);
-    // Every LDS read of the last exchange completed before the barrier that
-    // ends it, so the spectrum can be parked immediately.
-#pragma unroll
-    for (int m = 0; m < P / RL; m++)
-#pragma unroll
-        for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
-    __syncthreads();
-    if (active) {
-        float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
-#pragma unroll
-        for (int i = 0; i <= P; i++) {
-            const int k = t + i * T;
-            if (k <= M && k < (int)kout) {
-                const float2 a = L[lds_idx(k & (M - 1))];
-                const float2 b = L[lds_idx((M - k) & (M - 1))];
-                const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
-                // O = (a - conj b) / (2i) = (0.5 (a.y + b.y), -0.5 (a.x - b.x))
-                const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
-                const float2 w = wn[k];
-                const float2 wo = cmul_tw<-1>(o, w);
-                st_io<(T >= 16)>(&dst[k], cadd(e, wo));
-            }
-        }
-    }
-}
-
-// z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
-//   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
-//   X[k] = E + W_n^k O            (Z[M] == Z[0]).
 template <int I>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restrict__ in,
                                                            float* __restrict__ out,
@@ -353,7 +225,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
                                                            uint32_t ostride) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
     __shared__ float2 lds[B * lds_len(M)];
-    r2c_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, kout, ostride, blockIdx.x, lds);
+    r2c_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, kout, ostride, blockIdx.x, lds, threadIdx.x);
 }
 
 // ------------------------------------------------------------------ C2R
@@ -361,56 +233,6 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
 // with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
 // Reads bins k < kin of each row at a row stride of istride floats; bins at
 // or above kin are zero (the full transform: kin = M+1, istride = 2M+2).
-template <int I, int WG>
-__device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __restrict__ out,
-                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
-                                         uint32_t batch, float scale, uint32_t kin, uint32_t istride,
-                                         uint32_t g, float2* lds) {
-    using PL = PlanOf<I>;
-    constexpr int M = PL::N, T = PL::T, B = WG / T;
-    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
-    static_assert(WG % T == 0, "a work unit holds whole transforms");
-    const int tid = threadIdx.x;
-    const int bl = tid / T, t = tid % T;
-    const uint64_t tr = (uint64_t)g * B + bl;
-    const bool active = tr < batch;
-    float2* L = lds + bl * lds_len(M);
-    if (active) {
-        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)istride);
-#pragma unroll
-        for (int i = 0; i <= P; i++) {
-            const int k = t + i * T;
-            if (k <= M) L[k] = k < (int)kin ? ld_io<(T >= 16)>(&src[k]) : make_float2(0.f, 0.f);
-        }
-    }
-    __syncthreads();
-    float2 v[P];
-#pragma unroll
-    for (int m = 0; m < P / R0; m++)
-#pragma unroll
-        for (int r = 0; r < R0; r++) {
-            const int k = t + m * T + r * (M / R0);
-            float2 a = L[k], b = L[M - k];
-            if (k == 0) { a.y = 0.f; b.y = 0.f; }
-            const float2 s = make_float2(a.x + b.x, a.y - b.y);
-            const float2 d = make_float2(a.x - b.x, a.y + b.y);
-            const float2 wd = cmul_tw<1>(d, wn[k]);  // W_n^{-k} d
-            v[m * R0 + r] = make_float2(s.x - wd.y, s.y + wd.x);
-        }
-    __syncthreads();
-    fft_local<I, T, 1>(v, L, tw, t);
-    if (!active) return;
-    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M));
-#pragma unroll
-    for (int m = 0; m < P / RL; m++)
-#pragma unroll
-        for (int r = 0; r < RL; r++) {
-            const float2 z = v[m * RL + r];
-            st_io<(T >= 16)>(&dst[t + m * T + r * (M / RL)], make_float2(z.x * scale, z.y * scale));
-        }
-}
-
 template <int I>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restrict__ in,
                                                            float* __restrict__ out,
@@ -420,7 +242,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restri
                                                            uint32_t istride) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
     __shared__ float2 lds[B * lds_len(M)];
-    c2r_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, scale, kin, istride, blockIdx.x, lds);
+    c2r_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, scale, kin, istride, blockIdx.x, lds, threadIdx.x);
 }
 
 // ------------------------------------------------------------- long C2C
@@ -603,6 +425,17 @@ hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, 
     return hipGetLastError();
 }
 
+template <int I>
+hipError_t launch_c2c_single(const float2* in, float2* out, const float2* tw, uint32_t batch,
+                             int sign, hipStream_t s) {
+    constexpr int T = PlanOf<I>::T;
+    if (sign < 0)
+        hipLaunchKernelGGL((k_c2c_single<I, -1>), dim3(batch), dim3(T), 0, s, in, out, tw, batch);
+    else
+        hipLaunchKernelGGL((k_c2c_single<I, 1>), dim3(batch), dim3(T), 0, s, in, out, tw, batch);
+    return hipGetLastError();
+}
+
 template <int I1, int I2, int DIR>
 hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const float2* tw1,
                            const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s) {
@@ -633,6 +466,7 @@ namespace bfly_kern {
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s) {
+    if (variant == BFLY_VARIANT_SPLIT_MFMA) return c2c_mfma(in, out, n, batch, sign, table, s);
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     const float2* tw = reinterpret_cast<const float2*>(table);
@@ -649,6 +483,13 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
         default: break;
     }
     const bfly_host::TableLayout L = bfly_host::table_layout(0, n);
+    if (variant != BFLY_VARIANT_FOUR_STEP) {
+        switch (n) {
+            case 8192: return launch_c2c_single<idx(8192)>(i2, o2, tw + L.single, batch, sign, s);
+            case 16384: return launch_c2c_single<idx(16384)>(i2, o2, tw + L.single, batch, sign, s);
+            default: break;
+        }
+    }
     const float2 *t1 = tw + L.stage0, *t2 = tw + L.stage1, *wN = tw + L.extra;
     float2* ws = reinterpret_cast<float2*>(work);
     switch (n) {
@@ -682,8 +523,9 @@ hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batc
 hipError_t copy(const void* src, void* dst, size_t bytes, hipStream_t s) {
     const size_t n4 = bytes / 16;
     int dev = 0, cus = 256;
-    hipGetDevice(&dev);
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess)
+        cus = 256;
     size_t blocks = (n4 + 255) / 256;
     const size_t cap = (size_t)cus * 16;
     if (blocks > cap) blocks = cap;

@@ -43,6 +43,16 @@ constexpr Plan kPlans[] = {
 };
 constexpr int kNumPlans = (int)(sizeof(kPlans) / sizeof(kPlans[0]));
 
+// Radix schedules of one plan: 0 runs the radices as listed (radix 16 first,
+// the remainder radix last), 1 moves the remainder radix to the front.  Both
+// use the same radices and differ only in pass order (and so in rounding).
+constexpr int kSchedules = 2;
+constexpr int radix_at(int i, int sched, int s) {
+    return (sched == 0 || kPlans[i].nrad == 1)
+               ? kPlans[i].rad[s]
+               : (s == 0 ? kPlans[i].rad[kPlans[i].nrad - 1] : kPlans[i].rad[s - 1]);
+}
+
 constexpr int plan_index(uint32_t n) {
     for (int i = 0; i < kNumPlans; i++)
         if ((uint32_t)kPlans[i].n == n) return i;
@@ -51,11 +61,11 @@ constexpr int plan_index(uint32_t n) {
 
 // Complex entries of the inter-pass twiddle table of plan i:
 // sum over passes s >= 1 of Ns * R_s, Ns = R_0 * ... * R_{s-1}.
-constexpr size_t stage_twiddle_len(int i) {
+constexpr size_t stage_twiddle_len(int i, int sched = 0) {
     size_t len = 0, ns = 1;
     for (int s = 0; s < kPlans[i].nrad; s++) {
-        if (s > 0) len += ns * (size_t)kPlans[i].rad[s];
-        ns *= (size_t)kPlans[i].rad[s];
+        if (s > 0) len += ns * (size_t)radix_at(i, sched, s);
+        ns *= (size_t)radix_at(i, sched, s);
     }
     return len;
 }

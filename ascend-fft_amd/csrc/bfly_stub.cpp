@@ -8,6 +8,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <new>
 
 #include "bfly_hip.h"
 #include "bfly_tables.hpp"
@@ -94,11 +95,11 @@ int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
 }
 int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                 const float* table, void* work, void* stream) {
-    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) {
-        if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return e;
-        return BFLY_ERR_SHAPE;
-    }
-    return bfly_c2c(in, out, n, batch, sign, table, work, stream);
+    (void)stream;
+    if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work, variant))
+        return e;
+    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    return launch();
 }
 int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
     if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;
@@ -144,6 +145,64 @@ int bfly_probe_device(int device, char* json, size_t cap) {
     if (!json || cap == 0 || device != 0) return BFLY_ERR_SHAPE;
     const int w = std::snprintf(json, cap, "{\"name\": \"cpu-stub\", \"gcn_arch\": \"none\"}");
     return (w > 0 && (size_t)w < cap) ? BFLY_OK : BFLY_ERR_SHAPE;
+}
+
+
+// ------------------------------------------------------- grouped launch
+// The host plan (validation, unit list) is the library's own; the device
+// part is a host allocation standing in for the request table.
+namespace {
+struct StubGroup {
+    uint32_t nreq = 0, units = 0, grid = 0;
+    size_t bytes = 0;
+};
+}  // namespace
+
+int bfly_group_supported(int kind, uint32_t n, int workgroup) {
+    return bfly_host::group_ok(kind, n, workgroup) ? 1 : 0;
+}
+int bfly_group_create(const bfly_group_request* reqs, int nreq, const bfly_group_config* cfg,
+                      void** group) {
+    if (!group) return BFLY_ERR_SHAPE;
+    *group = nullptr;
+    bfly_host::GroupPlan P;
+    if (int e = bfly_host::group_plan(reqs, nreq, cfg, &P)) return e;
+    ++g.allocations;
+    if (g.allocations == g.fail_allocation) return BFLY_ERR_ALLOC;
+    StubGroup* G = new (std::nothrow) StubGroup();
+    if (!G) return BFLY_ERR_ALLOC;
+    G->nreq = (uint32_t)nreq;
+    G->units = (uint32_t)P.total;
+    const uint64_t cap = P.cfg.grid > 0 ? (uint64_t)P.cfg.grid : 512u;  // 256 CUs x 2 workgroups
+    G->grid = (uint32_t)(cap < P.total ? cap : P.total);
+    G->bytes = 48u * (size_t)nreq + 8u * (size_t)P.total;
+    ++g.copies;
+    if (g.copies == g.fail_copy) {
+        delete G;
+        ++g.frees;
+        return BFLY_ERR_H2D;
+    }
+    *group = G;
+    return BFLY_OK;
+}
+int bfly_group_launch(void* group, unsigned long long*, void*) {
+    if (!group) return BFLY_ERR_SHAPE;
+    return launch();
+}
+int bfly_group_query(void* group, uint32_t* units, uint32_t* grid, size_t* bytes) {
+    const StubGroup* G = static_cast<const StubGroup*>(group);
+    if (!G) return BFLY_ERR_SHAPE;
+    if (units) *units = G->units;
+    if (grid) *grid = G->grid;
+    if (bytes) *bytes = G->bytes;
+    return BFLY_OK;
+}
+int bfly_group_destroy(void* group) {
+    if (group) {
+        ++g.frees;
+        delete static_cast<StubGroup*>(group);
+    }
+    return BFLY_OK;
 }
 
 }  // extern "C"

@@ -4,6 +4,7 @@
 //
 // Twiddles are evaluated in FP64 from the exact rational angle 2*pi*m/L
 // (integer m reduced modulo L first) and rounded once to fp32.
+#include <algorithm>
 #include <cmath>
 #include <cstddef>
 #include <cstdint>
@@ -31,20 +32,21 @@ bool shape_ok(int kind, uint32_t n) {
 }
 
 // FP64 e^{-2 pi i m / L} rounded to fp32.
-static inline void twiddle(uint64_t m, uint64_t L, float* out) {
+void twiddle(uint64_t m, uint64_t L, float* out) {
     m %= L;
     const double ang = -2.0 * M_PI * (double)m / (double)L;
     out[0] = (float)std::cos(ang);
     out[1] = (float)std::sin(ang);
 }
 
-// Inter-pass table of local plan i, complex entries at [k * R_s + r].
-static size_t fill_stage(int i, float* dst) {
+// Inter-pass table of local plan i under radix schedule `sched`, complex
+// entries at [k * R_s + r].
+size_t fill_stage(int i, float* dst, int sched) {
     const auto& p = bfly_plan::kPlans[i];
     size_t off = 0;
     uint64_t ns = 1;
     for (int s = 0; s < p.nrad; s++) {
-        const uint64_t R = (uint64_t)p.rad[s];
+        const uint64_t R = (uint64_t)bfly_plan::radix_at(i, sched, s);
         if (s > 0) {
             for (uint64_t k = 0; k < ns; k++)
                 for (uint64_t r = 0; r < R; r++) {
@@ -112,6 +114,13 @@ int fill_table(int kind, uint32_t n, float* dst) {
 
 // Largest grid any launcher issues for (kind, n, batch) must fit a 32-bit x
 // dimension.
+bool split_mfma_ok(uint32_t n) { return n == 1024 || n == 2048 || n == 4096; }
+
+bool needs_work(int kind, uint32_t n, int variant) {
+    return kind == BFLY_KIND_C2C && n > (uint32_t)bfly_plan::kMaxLocalN &&
+           (n > (uint32_t)bfly_plan::kMaxSingleN || variant == BFLY_VARIANT_FOUR_STEP);
+}
+
 static bool grid_ok(int kind, uint32_t n, uint32_t batch) {
     uint64_t blocks = 0;
     const uint64_t b = batch;
@@ -122,6 +131,8 @@ static bool grid_ok(int kind, uint32_t n, uint32_t batch) {
         const uint64_t c2 = bfly_plan::kBlock / bfly_plan::threads_per_transform(n2);
         const uint64_t g1 = b * (n2 / c1), g2 = b * (n1 / c2);
         blocks = g1 > g2 ? g1 : g2;
+        // single-workgroup path: one workgroup of n/16 threads per transform
+        if (n <= (uint32_t)bfly_plan::kMaxSingleN && b * (n / 16) > 0xffffffffull) return false;
     } else {
         const uint32_t local = kind == BFLY_KIND_C2C ? n : n / 2;
         const uint64_t per = bfly_plan::kBlock / bfly_plan::threads_per_transform(local);
@@ -131,12 +142,14 @@ static bool grid_ok(int kind, uint32_t n, uint32_t batch) {
 }
 
 int check_execute(int kind, int sign, const void* in, const void* out, uint32_t n,
-                  uint32_t batch, const void* table, const void* work) {
+                  uint32_t batch, const void* table, const void* work, int variant) {
     if (!in || !out || batch == 0 || !shape_ok(kind, n)) return BFLY_ERR_SHAPE;
     if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
     if (!grid_ok(kind, n, batch)) return BFLY_ERR_SHAPE;
     if (!table) return BFLY_ERR_TABLE;
-    if (kind == BFLY_KIND_C2C && n > (uint32_t)bfly_plan::kMaxLocalN && !work) return BFLY_ERR_SHAPE;
+    if (needs_work(kind, n, variant) && !work) return BFLY_ERR_SHAPE;
+    if (kind == BFLY_KIND_C2C && variant == BFLY_VARIANT_SPLIT_MFMA && !split_mfma_ok(n))
+        return BFLY_ERR_SHAPE;
     return BFLY_OK;
 }
 
@@ -156,6 +169,75 @@ bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride) {
     return bins >= 1 && bins <= n / 2 + 1 && stride >= 2 * bins && (stride % 2) == 0;
 }
 
+bool group_ok(int kind, uint32_t n, int wg) {
+    if (wg != 512 && wg != 1024) return false;
+    if (!pow2(n)) return false;
+    const uint32_t m = group_local_len(kind, n);
+    if (kind == BFLY_KIND_C2C) {
+        if (m < 16) return false;
+    } else if (kind == BFLY_KIND_R2C || kind == BFLY_KIND_C2R) {
+        if (n < 32 || m > (uint32_t)bfly_plan::kMaxLocalN) return false;
+    } else {
+        return false;
+    }
+    return bfly_plan::plan_index(m) >= 0 && m / bfly_plan::kPoints <= (uint32_t)wg;
+}
+
+int group_plan(const bfly_group_request* reqs, int nreq, const bfly_group_config* cfg, GroupPlan* out) {
+    if (!reqs || nreq <= 0 || !out) return BFLY_ERR_SHAPE;
+    bfly_group_config c{512, 0, 0, 0, 0};
+    if (cfg) c = *cfg;
+    if (c.workgroup == 0) c.workgroup = 512;
+    if ((c.workgroup != 512 && c.workgroup != 1024) || (c.schedule != 0 && c.schedule != 1) ||
+        c.order < 0 || c.order > 2 || (c.dynamic != 0 && c.dynamic != 1) || c.grid < 0)
+        return BFLY_ERR_SHAPE;
+    GroupPlan& P = *out;
+    P = GroupPlan{};
+    P.cfg = c;
+    P.per.resize(nreq);
+    P.count.resize(nreq);
+    for (int r = 0; r < nreq; r++) {
+        const bfly_group_request& q = reqs[r];
+        if (!q.in || !q.out || q.batch == 0 || !group_ok(q.kind, q.n, c.workgroup)) return BFLY_ERR_SHAPE;
+        if (q.kind == BFLY_KIND_C2C && q.sign != -1 && q.sign != 1) return BFLY_ERR_SHAPE;
+        P.per[r] = (uint32_t)c.workgroup / (group_local_len(q.kind, q.n) / bfly_plan::kPoints);
+        P.count[r] = (uint32_t)(((uint64_t)q.batch + P.per[r] - 1) / P.per[r]);
+        P.total += P.count[r];
+    }
+    if (P.total == 0 || P.total > 0x7fffffffull) return BFLY_ERR_SHAPE;
+    P.unit_req.reserve(P.total);
+    P.unit_grp.reserve(P.total);
+    auto push = [&](int r, uint32_t g) {
+        P.unit_req.push_back((uint32_t)r);
+        P.unit_grp.push_back(g);
+    };
+    if (c.order == 2) {
+        // round robin: unit g of every request before unit g+1 of any
+        uint32_t most = 0;
+        for (int r = 0; r < nreq; r++) most = std::max(most, P.count[r]);
+        for (uint32_t g = 0; g < most; g++)
+            for (int r = 0; r < nreq; r++)
+                if (g < P.count[r]) push(r, g);
+        return BFLY_OK;
+    }
+    std::vector<int> ord(nreq);
+    for (int r = 0; r < nreq; r++) ord[r] = r;
+    if (c.order == 1) {
+        // most work per unit first: filled transforms * n' * log2 n'
+        auto work = [&](int r) {
+            const uint32_t m = group_local_len(reqs[r].kind, reqs[r].n);
+            const uint64_t full = std::min<uint64_t>(reqs[r].batch, P.per[r]);
+            uint32_t lg = 0;
+            while ((1u << lg) < m) lg++;
+            return full * m * (uint64_t)lg;
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return work(a) > work(b); });
+    }
+    for (int r : ord)
+        for (uint32_t g = 0; g < P.count[r]; g++) push(r, g);
+    return BFLY_OK;
+}
+
 }  // namespace bfly_host
 
 extern "C" {
@@ -171,8 +253,7 @@ int bfly_table_fill(int kind, uint32_t n, float* host_table) {
 }
 
 size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch) {
-    if (kind != BFLY_KIND_C2C || !bfly_host::shape_ok(kind, n) ||
-        n <= (uint32_t)bfly_plan::kMaxLocalN)
+    if (!bfly_host::shape_ok(kind, n) || !bfly_host::needs_work(kind, n, BFLY_VARIANT_AUTO))
         return 0;
     return (size_t)n * (size_t)batch * 8u;
 }

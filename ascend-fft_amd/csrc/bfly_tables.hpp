@@ -3,6 +3,9 @@
 #pragma once
 #include <cstddef>
 #include <cstdint>
+#include <vector>
+
+#include "bfly_hip.h"
 
 namespace bfly_host {
 
@@ -21,16 +24,49 @@ struct TableLayout {
 };
 
 bool shape_ok(int kind, uint32_t n);
+// FP64 e^{-2 pi i m / L} (m reduced modulo L) rounded to fp32 into out[0..1].
+void twiddle(uint64_t m, uint64_t L, float* out);
+// Writes the inter-pass twiddle table of plan `plan` under radix schedule
+// `sched` (interleaved fp32, when dst is not null); returns its complex length.
+size_t fill_stage(int plan, float* dst, int sched = 0);
 TableLayout table_layout(int kind, uint32_t n);
 int fill_table(int kind, uint32_t n, float* dst);
 // Argument guards of the execute entry points: BFLY_OK or the status code.
+// `variant` matters only for long C2C, where it decides whether a workspace
+// is needed (BFLY_VARIANT_FOUR_STEP, or n above the single-workgroup limit).
 int check_execute(int kind, int sign, const void* in, const void* out, uint32_t n,
-                  uint32_t batch, const void* table, const void* work);
+                  uint32_t batch, const void* table, const void* work,
+                  int variant = BFLY_VARIANT_AUTO);
+// True when a C2C of length n run as `variant` goes through the four-step
+// path and its n*batch complex workspace.
+bool needs_work(int kind, uint32_t n, int variant);
+// Lengths the split-FP16 matrix-core variant runs (plans [16, 16, R]).
+bool split_mfma_ok(uint32_t n);
 
 // Mode-truncated real transforms: 1 <= bins <= n/2+1, stride (floats) even
 // and at least 2*bins.
 bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride);
 // The variant BFLY_VARIANT_AUTO resolves to for a C2C of (n, batch).
 int auto_variant(uint32_t n, uint32_t batch);
+
+// ------------------------------------------------------- grouped launch
+// Whether a grouped launch with `wg` threads per workgroup takes (kind, n).
+bool group_ok(int kind, uint32_t n, int wg);
+// Points per transform of the complex plan a request runs (n, or n/2 for
+// the real kinds).
+inline uint32_t group_local_len(int kind, uint32_t n) { return kind == BFLY_KIND_C2C ? n : n / 2; }
+
+// Host side of a grouped launch: the resolved configuration, transforms per
+// unit and unit count of every request, and the unit list (request, group
+// index) in launch order.
+struct GroupPlan {
+    bfly_group_config cfg{};
+    std::vector<uint32_t> per, count;
+    std::vector<uint32_t> unit_req, unit_grp;
+    uint64_t total = 0;
+};
+// Validates requests and configuration and builds the unit list: BFLY_OK or
+// BFLY_ERR_SHAPE.
+int group_plan(const bfly_group_request* reqs, int nreq, const bfly_group_config* cfg, GroupPlan* out);
 
 }  // namespace bfly_host

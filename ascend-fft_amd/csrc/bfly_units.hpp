@@ -1,0 +1,220 @@
+// bfly_units.hpp -- the per-workgroup work units of the gfx950 kernels.
+//
+// A work unit is B = WG/T consecutive transforms of one batch (T = n/16
+// threads each) taken through one HBM read and one HBM write.  The one-shot
+// kernels run one unit per workgroup; the grouped kernel runs units of many
+// requests from one persistent launch.  Both call these functions, so a
+// transform computes the same bits whichever kernel runs it.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <utility>
+
+#include "bfly_device.hpp"
+#include "bfly_plans.hpp"
+
+namespace bfly_dev {
+
+using bfly_plan::kPlans;
+
+// Plan I under radix schedule SC (bfly_plan::radix_at).
+template <int I, int SC = 0>
+struct PlanOf {
+    static constexpr int N = kPlans[I].n;
+    static constexpr int NRAD = kPlans[I].nrad;
+    static constexpr int R0 = bfly_plan::radix_at(I, SC, 0);
+    static constexpr int RL = bfly_plan::radix_at(I, SC, NRAD - 1);
+    static constexpr int P = bfly_plan::kPoints;
+    static constexpr int T = N / P;  // threads per transform
+};
+
+template <int I, int T, int DIR, int LX, int SC = 0, size_t... S>
+__device__ __forceinline__ void run_plan(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
+                                         const float2* __restrict__ tw, int t,
+                                         std::index_sequence<S...>) {
+    stockham<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, bfly_plan::radix_at(I, SC, S)...>(
+        v, lds, rb, tw, t);
+}
+
+template <int I, int T, int DIR, int LX, size_t... S>
+__device__ __forceinline__ void run_plan_rt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
+                                            const float2* tr, int t, std::index_sequence<S...>) {
+    stockham_rt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tr, t);
+}
+
+template <int I, int T, int DIR, int LX, size_t... S>
+__device__ __forceinline__ void run_plan_lt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
+                                            const float2* tlo, const float2* thi, int t,
+                                            std::index_sequence<S...>) {
+    stockham_lt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tlo, thi, t);
+}
+
+template <int I, int T, size_t... S>
+__device__ __forceinline__ void load_plan_twiddles(float2* tr, const float2* __restrict__ tw, int t,
+                                                   std::index_sequence<S...>) {
+    load_twiddles<kPlans[I].n, bfly_plan::kPoints, T, 1, kPlans[I].rad[S]...>(tr, tw, t);
+}
+
+template <int I, size_t... S>
+constexpr int plan_twiddle_regs(std::index_sequence<S...>) {
+    return TwRegCount<bfly_plan::kPoints, 1, kPlans[I].rad[S]...>::value;
+}
+
+// Padded-region form used by the real and long kernels: `lds` is the
+// transform's own region.
+template <int I, int T, int DIR, int SC = 0>
+__device__ __forceinline__ void fft_local(float2 (&v)[bfly_plan::kPoints], float2* lds,
+                                          const float2* __restrict__ tw, int t) {
+    run_plan<I, T, DIR, LX_PAD, SC>(v, lds, 0, tw, t, std::make_index_sequence<kPlans[I].nrad>{});
+}
+
+// ------------------------------------------------------------------ C2C
+// One work unit: the WG/T transforms g*B .. g*B+B-1 of a batch (B = WG/T),
+// one HBM read and one HBM write per element.  `lds` holds B regions; `tid`
+// is the thread's index in the workgroup.
+template <int I, int DIR, int LX, int WG, int SC = 0>
+__device__ __forceinline__ void c2c_unit(const float2* __restrict__ in, float2* __restrict__ out,
+                                         const float2* __restrict__ tw, uint32_t batch, uint32_t g,
+                                         float2* lds, int tid) {
+    using PL = PlanOf<I, SC>;
+    constexpr int N = PL::N, T = PL::T, B = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    const bool active = tr < batch;
+    float2 v[P];
+    if (active) {
+        const float2* src = in + tr * N;
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
+    }
+    run_plan<I, T, DIR, LX, SC>(v, LX == LX_SWZ ? lds : lds + bl * lds_len(N),
+                                LX == LX_SWZ ? bl * N : 0, tw, t, std::make_index_sequence<PL::NRAD>{});
+    if (active) {
+        float2* dst = out + tr * N;
+#pragma unroll
+        for (int m = 0; m < P / RL; m++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], v[m * RL + r]);
+    }
+}
+
+// ------------------------------------------------------------------ R2C
+// z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
+//   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
+//   X[k] = E + W_n^k O            (Z[M] == Z[0]).
+// Writes bins k < kout of each row at a row stride of ostride floats (the
+// full transform: kout = M+1, ostride = 2M+2; a mode-truncated one writes
+// only its first kout bins).
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __restrict__ out,
+                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                         uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
+                                         float2* lds, int tid) {
+    using PL = PlanOf<I, SC>;
+    constexpr int M = PL::N, T = PL::T, B = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    const bool active = tr < batch;
+    float2* L = lds + bl * lds_len(M);
+    float2 v[P];
+    if (active) {
+        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)(2 * M));
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (M / R0)]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
+    }
+    fft_local<I, T, -1, SC>(v, L, tw, t);
+    // Every LDS read of the last exchange completed before the barrier that
+    // ends it, so the spectrum can be parked immediately.
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
+    __syncthreads();
+    if (active) {
+        float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
+#pragma unroll
+        for (int i = 0; i <= P; i++) {
+            const int k = t + i * T;
+            if (k <= M && k < (int)kout) {
+                const float2 a = L[lds_idx(k & (M - 1))];
+                const float2 b = L[lds_idx((M - k) & (M - 1))];
+                const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+                // O = (a - conj b) / (2i) = (0.5 (a.y + b.y), -0.5 (a.x - b.x))
+                const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+                const float2 w = wn[k];
+                const float2 wo = cmul_tw<-1>(o, w);
+                st_io<(T >= 16)>(&dst[k], cadd(e, wo));
+            }
+        }
+    }
+}
+
+// ------------------------------------------------------------------ C2R
+// Z'[k] = (X[k] + conj X[M-k]) + i W_n^{-k} (X[k] - conj X[M-k]), k < M,
+// with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
+// Reads bins k < kin of each row at a row stride of istride floats; bins at
+// or above kin are zero (the full transform: kin = M+1, istride = 2M+2).
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __restrict__ out,
+                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                         uint32_t batch, float scale, uint32_t kin, uint32_t istride,
+                                         uint32_t g, float2* lds, int tid) {
+    using PL = PlanOf<I, SC>;
+    constexpr int M = PL::N, T = PL::T, B = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    const bool active = tr < batch;
+    float2* L = lds + bl * lds_len(M);
+    if (active) {
+        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)istride);
+#pragma unroll
+        for (int i = 0; i <= P; i++) {
+            const int k = t + i * T;
+            if (k <= M) L[k] = k < (int)kin ? ld_io<(T >= 16)>(&src[k]) : make_float2(0.f, 0.f);
+        }
+    }
+    __syncthreads();
+    float2 v[P];
+#pragma unroll
+    for (int m = 0; m < P / R0; m++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int k = t + m * T + r * (M / R0);
+            float2 a = L[k], b = L[M - k];
+            if (k == 0) { a.y = 0.f; b.y = 0.f; }
+            const float2 s = make_float2(a.x + b.x, a.y - b.y);
+            const float2 d = make_float2(a.x - b.x, a.y + b.y);
+            const float2 wd = cmul_tw<1>(d, wn[k]);  // W_n^{-k} d
+            v[m * R0 + r] = make_float2(s.x - wd.y, s.y + wd.x);
+        }
+    __syncthreads();
+    fft_local<I, T, 1, SC>(v, L, tw, t);
+    if (!active) return;
+    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M));
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const float2 z = v[m * RL + r];
+            st_io<(T >= 16)>(&dst[t + m * T + r * (M / RL)], make_float2(z.x * scale, z.y * scale));
+        }
+}
+
+}  // namespace bfly_dev

@@ -195,7 +195,8 @@ bool Mapping::validate(const Hardware& hw, std::string* why) const {
 // ==================================================================== P
 // LDS bytes of one 256-thread workgroup: 256/T transforms (T = n/P threads
 // each) of lds_len(n) = n + n/16 complex, i.e. 256 * P * 8 * 17/16 for every
-// n <= 4096; long lengths take the larger of the column and row kernels
+// n <= 4096; 8192 and 16384 hold one transform per workgroup (68 / 136
+// KiB); longer lengths take the larger of the column and row kernels
 // (C columns of lds_len(N_i) + 1 complex each).
 size_t StagePlan::ubBytes(uint32_t n) const {
     const size_t P = (size_t)std::max(radix, 2);
@@ -203,7 +204,7 @@ size_t StagePlan::ubBytes(uint32_t n) const {
         const size_t T = std::max<size_t>(1, m / P), B = std::max<size_t>(1, 256 / T);
         return B * ((size_t)m + m / 16) * 8;
     };
-    if (n <= (uint32_t)bfly_plan::kMaxLocalN) return local(n);
+    if (n <= (uint32_t)bfly_plan::kMaxSingleN) return local(n);
     uint32_t n1 = 0, n2 = 0;
     if (!bfly_plan::long_split(n, &n1, &n2)) return local(n);
     auto column = [&](uint32_t m) {
@@ -226,7 +227,7 @@ Metric estimate(const Hardware& hw, const Mapping& a, const StagePlan& p, uint32
     const double launchUs = 4.0;
     const double clockGHz = 2.4, valuEff = 0.38;
     const int cus = std::min(std::max(a.udCore, 1), std::max(hw.aicoreNum, 1));
-    const double roundTrips = n > (uint32_t)bfly_plan::kMaxLocalN ? 2.0 : 1.0;
+    const double roundTrips = n > (uint32_t)bfly_plan::kMaxSingleN ? 2.0 : 1.0;
     const double bytes = 16.0 * n * (double)batch * roundTrips;
     const double memUs = bytes / (hw.hbmCopyGBs * 1e3) * ((double)hw.aicoreNum / cus);
     const double flops = 5.0 * n * ilog2(n) * (double)batch;

@@ -12,6 +12,11 @@ the whole batch ([4096][8192] float32, interleaved), inputs resident in HBM.  Pr
   chosen         every kernel variant of this shape timed on the same buffers
                  and gated against FP64; the fastest passing one, its speedup
                  over the plain one-shot variant and its relative error
+  patterns       the uneven batched workload (512 requests) as one grouped
+                 persistent launch under hipGraph vs per-request launches:
+                 transforms/s, GB/s, p50/p99 per-request latency; the
+                 spectral-convolution layer (fwd+bwd) vs the same layer on
+                 torch.fft
 
 Timed steps rotate over several input/output buffer pairs (>= 1 GiB in
 total) so that the 256 MB last-level cache cannot serve one step's input from
@@ -45,6 +50,8 @@ def parse():
     ap.add_argument("--rotate", type=int, default=4, help="input/output buffer pairs cycled")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-patterns", action="store_true",
+                    help="skip the grouped uneven-batch measurement")
     ap.add_argument("--probe-copy", action="store_true",
                     help="also run the streaming copy kernel on 256 MiB (counter calibration)")
     return ap.parse_args()
@@ -90,6 +97,114 @@ def cpu_baseline(budget_s: float):
         "sample": f"{done} transforms ({done / BATCH:.2f} passes over the 4096 seeded rows) of "
                   f"the N=4096 C2C workload through the reference's refFftF32 (FP64, one "
                   f"thread) in {dt:.1f} s",
+    }
+
+
+def grouped_pattern(reps: int = 20):
+    """The uneven batched workload: 512 C2C/R2C requests (n log-uniform over
+    64..8192, batch U[1,256], seed 3) as ONE grouped persistent launch
+    captured in a hipGraph, configuration chosen by the autotuner, against
+    the same requests as 512 per-request launches (eager and captured).
+    Outputs are checked against the per-request launches before timing."""
+    import numpy as np
+    import torch
+    from bfly import group
+
+    specs = group.uneven_specs()
+    reqs = group.make_requests(specs, seed=3)
+    group.launch_per_request(reqs)
+    torch.cuda.synchronize()
+    ref = group.outputs(reqs)
+    tuned = group.autotune(reqs, reps=5, reference=ref)
+    g = group.Group(reqs, tuned.best)
+    for r in reqs:
+        r.y.fill_(float("nan"))
+    g.launch()
+    torch.cuda.synchronize()
+    err = group.max_rel_diff(group.outputs(reqs), ref)
+    graph = group.capture(g.launch)
+    ms_group = group.time_ms(graph.replay, reps=reps)
+    per_graph = group.capture(lambda: group.launch_per_request(reqs))
+    ms_per_graph = group.time_ms(per_graph.replay, reps=reps)
+    ms_per_eager = group.time_ms(lambda: group.launch_per_request(reqs), reps=max(3, reps // 4))
+    lat = group.latencies_us(g)
+    lat_seq = group.per_request_latencies_us(reqs)
+    nbytes = g.algorithmic_bytes()
+    transforms = g.transforms()
+    kinds = [k for k, _, _ in specs]
+    out = {
+        "workload": "512 requests, n log-uniform 64..8192, batch U[1,256], C2C/R2C 50/50, seed 3",
+        "requests": len(reqs),
+        "c2c_requests": int(sum(1 for k in kinds if k == 0)),
+        "transforms": transforms,
+        "algorithmic_bytes": nbytes,
+        "config": tuned.best.name(),
+        "units": g.units,
+        "grid": g.grid,
+        "ms_grouped_graph": round(ms_group, 4),
+        "transforms_per_s": round(transforms / (ms_group * 1e-3), 1),
+        "GB_per_s": round(nbytes / (ms_group * 1e-3) / 1e9, 1),
+        "pct_hbm_peak": round(100.0 * nbytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
+        "latency_us_p50": round(float(np.percentile(lat, 50)), 2),
+        "latency_us_p99": round(float(np.percentile(lat, 99)), 2),
+        "per_request_launches": {
+            "ms_graph": round(ms_per_graph, 4),
+            "ms_eager": round(ms_per_eager, 4),
+            "latency_us_p50": round(float(np.percentile(lat_seq, 50)), 2),
+            "latency_us_p99": round(float(np.percentile(lat_seq, 99)), 2),
+        },
+        "speedup_vs_per_request_graph": round(ms_per_graph / ms_group, 3),
+        "max_rel_diff_vs_per_request": err,
+        "tuned_candidates": [{"config": n, "ms": round(ms, 4), "rel_diff": e}
+                             for n, ms, e, _, _ in tuned.table],
+    }
+    del graph, per_graph
+    g.close()
+    return out
+
+
+def spectral_pattern(reps: int = 10):
+    """The 1-D spectral-convolution layer, x [64, 128, 4096] ~ N(0,1),
+    W [128, 128, 256] complex ~ N(0, 1/256) per part (seed 4): forward +
+    backward on the mode-truncated kernels vs the same layer on torch.fft
+    (hipFFT), outputs and gradients compared in FP64."""
+    import torch
+    from bfly.spectral import spectral_conv1d, torch_fft_reference
+
+    torch.manual_seed(4)
+    B, C, N, K = 64, 128, 4096, 256
+    x = torch.randn(B, C, N, device="cuda")
+    s = (1.0 / (C * 2)) ** 0.5
+    w = torch.complex(torch.randn(C, C, K, device="cuda") * s, torch.randn(C, C, K, device="cuda") * s)
+    up = torch.randn(B, C, N, device="cuda")
+
+    def run(layer):
+        xa, wa = x.clone().requires_grad_(True), w.clone().requires_grad_(True)
+        y = layer(xa, wa)
+        (y * up).sum().backward()
+        return y.detach(), xa.grad, wa.grad
+
+    def timed(layer):
+        for _ in range(2):
+            run(layer)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            run(layer)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    ours, ref = run(spectral_conv1d), run(torch_fft_reference)
+    rel = [float((a.double() - b.double()).norm() / b.double().norm()) for a, b in zip(ours, ref)]
+    ms_ours, ms_ref = timed(spectral_conv1d), timed(torch_fft_reference)
+    return {
+        "workload": "spectral conv1d fwd+bwd, x [64,128,4096], W [128,128,256] complex, seed 4",
+        "ms_fwd_bwd": round(ms_ours, 3),
+        "ms_fwd_bwd_torch_fft": round(ms_ref, 3),
+        "speedup_vs_torch_fft": round(ms_ref / ms_ours, 3),
+        "rel_l2_vs_torch_fft": {"y": rel[0], "dx": rel[1], "dw": rel[2]},
     }
 
 
@@ -186,6 +301,8 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
     }
+    if not args.no_patterns:
+        line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern()}
     print(json.dumps(line))
 
 

@@ -85,9 +85,11 @@ BFLY_API int bfly_supported(int kind, uint32_t n);
 BFLY_API size_t bfly_table_floats(int kind, uint32_t n);
 BFLY_API int bfly_table_fill(int kind, uint32_t n, float* host_table);
 
-/* Device workspace (bytes) an execute call needs; 0 for every length that
- * runs in one kernel.  Long C2C (n > 4096) needs n*batch complex values for
- * the four-step intermediate. */
+/* Device workspace (bytes) an execute call with BFLY_VARIANT_AUTO needs; 0
+ * for every length that runs in one kernel (C2C up to 16384, all real
+ * transforms).  C2C above 16384 runs four-step and needs n*batch complex
+ * values for the intermediate; so does BFLY_VARIANT_FOUR_STEP at 8192 and
+ * 16384. */
 BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
 
 /* --------------------------------------------------------------- execute
@@ -118,7 +120,18 @@ BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, i
  *                        register-resident twiddles
  *   BFLY_VARIANT_STREAM2 streaming form with two transforms in flight per
  *                        workgroup (swizzled LDS, register twiddles)
- * Long lengths (n > 4096) have one variant; every id is accepted for them. */
+ *   BFLY_VARIANT_FOUR_STEP  n = 8192 / 16384 through the two-kernel four-step
+ *                        path (needs n*batch*8 bytes of workspace) instead of
+ *                        one 512 / 1024-thread workgroup per transform
+ *   BFLY_VARIANT_SPLIT_MFMA n = 1024 / 2048 / 4096 only (else BFLY_ERR_SHAPE):
+ *                        the radix-16 passes as 16x16 complex DFT products on
+ *                        the matrix cores in split FP16 (hi + lo operands,
+ *                        FP32 accumulation, per-butterfly power-of-two
+ *                        scaling); FP32 twiddles.  Relative L2 error vs FP64
+ *                        about 2x that of the FP32 variants, within 1e-5
+ * For n <= 4096 FOUR_STEP means PLAIN; above 4096 every other id selects the
+ * length's default path (one workgroup per transform up to 16384, four-step
+ * above). */
 #define BFLY_VARIANT_AUTO 0
 #define BFLY_VARIANT_PLAIN 1
 #define BFLY_VARIANT_STREAM 2
@@ -126,7 +139,9 @@ BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, i
 #define BFLY_VARIANT_STREAM_SWZ 4
 #define BFLY_VARIANT_STREAM_LT 5
 #define BFLY_VARIANT_STREAM2 6
-#define BFLY_VARIANT_LAST 6
+#define BFLY_VARIANT_FOUR_STEP 7
+#define BFLY_VARIANT_SPLIT_MFMA 8
+#define BFLY_VARIANT_LAST 8
 BFLY_API int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
                          int variant, const float* table, void* work, void* stream);
 
@@ -153,6 +168,63 @@ BFLY_API int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch
                          uint32_t out_stride, const float* table, void* stream);
 BFLY_API int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                          uint32_t in_stride, const float* table, void* stream);
+
+/* ------------------------------------------------------- grouped launch
+ * Many independent transforms of different kinds, lengths and batch sizes
+ * run by ONE persistent kernel driven by a device-resident request table.
+ * No counterpart in the reference: its Plan::run executes one (n, batch)
+ * shape per launch (butterfly.cpp:723-749), so a caller with many uneven
+ * requests pays one launch and one tail per request.  The table is built and
+ * uploaded once by bfly_group_create; bfly_group_launch only launches, so it
+ * can be captured into a hipGraph and replayed (the table's device address
+ * stays fixed until bfly_group_destroy).  Results are those of per-request
+ * bfly_c2c_ex(PLAIN) / bfly_r2c / bfly_c2r calls: with schedule 0 the same
+ * kernels' per-transform code runs, bit for bit.
+ *
+ * Work is cut into units of 16*workgroup points (one HBM read and write
+ * each): a unit is workgroup/(n'/16) transforms of one request, n' = n for
+ * C2C and n/2 for the real kinds.  Lengths: C2C 16..8192 (16..16384 with
+ * 1024-thread workgroups), R2C/C2R 32..8192. */
+typedef struct bfly_group_request {
+    const void* in;   /* device input in the kind's execute layout */
+    void* out;        /* device output in the kind's execute layout */
+    int32_t kind;     /* BFLY_KIND_C2C / R2C / C2R */
+    int32_t sign;     /* C2C: -1 forward, +1 unnormalised inverse; else ignored */
+    uint32_t n;       /* transform length */
+    uint32_t batch;   /* transforms, >= 1 */
+} bfly_group_request;
+
+typedef struct bfly_group_config {
+    int32_t workgroup; /* threads per workgroup: 512 or 1024; 0 = 512 */
+    int32_t schedule;  /* radix schedule: 0 = radix 16 first, remainder radix last;
+                          1 = remainder radix first */
+    int32_t order;     /* unit order: 0 = request order; 1 = most work per unit
+                          first; 2 = round robin over requests */
+    int32_t dynamic;   /* 0 = workgroup w runs units w, w+grid, ...;
+                          1 = workgroups take units from an atomic counter */
+    int32_t grid;      /* workgroups launched; 0 = the resident capacity */
+} bfly_group_config;
+
+/* 1 when a grouped launch with `workgroup` threads (512 / 1024) takes
+ * (kind, n), else 0. */
+BFLY_API int bfly_group_supported(int kind, uint32_t n, int workgroup);
+/* Validates the requests, builds the request table, the unit list and the
+ * twiddle tables of every distinct length on the host and uploads them into
+ * one device allocation (hipMalloc + synchronous copy; call outside stream
+ * capture).  cfg may be NULL (all defaults).  *group receives the handle. */
+BFLY_API int bfly_group_create(const bfly_group_request* reqs, int nreq,
+                               const bfly_group_config* cfg, void** group);
+/* Launches the group on `stream`; allocates nothing, copies nothing.  When
+ * `stamps` is a device array of nreq+1 uint64, a small kernel first clears
+ * it, then the grouped kernel records stamps[0] = earliest workgroup start
+ * and stamps[1+r] = completion of request r's last unit, in ticks of the
+ * device wall clock (bfly_probe_device reports its rate).  Two launches of
+ * one group on different streams must not overlap (the dynamic schedule's
+ * counter is per group). */
+BFLY_API int bfly_group_launch(void* group, unsigned long long* stamps, void* stream);
+/* units = work units, grid = workgroups launched, bytes = device bytes held. */
+BFLY_API int bfly_group_query(void* group, uint32_t* units, uint32_t* grid, size_t* bytes);
+BFLY_API int bfly_group_destroy(void* group);
 
 /* ------------------------------------------------------------- probes */
 /* Device-to-device copy of `bytes` (16-byte aligned, multiple of 16) with

@@ -79,6 +79,63 @@ def test_c2c_every_variant(n, variant):
         gate_c2c(y.cpu().numpy(), x, n, sign)
 
 
+def _run_variant(x, n, b, sign, variant):
+    import torch
+    from bfly import _lib
+    xt = torch.from_numpy(np.ascontiguousarray(x)).cuda()
+    y = torch.empty_like(xt)
+    tab = _ops().table(0, n, xt.device)
+    rc = _lib.hip().bfly_c2c_ex(xt.data_ptr(), y.data_ptr(), n, b, sign, variant, tab.data_ptr(),
+                                None, torch.cuda.current_stream().cuda_stream)
+    assert rc == 0
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("n", [1024, 2048, 4096])
+def test_split_mfma_variant_error_vs_fp64(n):
+    # Radix-16 passes on the matrix cores in split FP16: same FP64 gates as
+    # the FP32 paths; its error is stated against FP64 and stays within a
+    # few times the FP32 Stockham's (about 1.3e-7 at these lengths).
+    from bfly import _lib
+    for b, sign in ((1, -1), (7, 1), (64, -1), (1000, 1)):
+        x = oracle.uniform_input(21, (b, 2 * n))
+        got = _run_variant(x, n, b, sign, _lib.VARIANT_SPLIT_MFMA)
+        e = gate_c2c(got, x, n, sign)
+        assert e < 2e-6, f"n={n} b={b} rel-L2 {e:.3e}"
+
+
+@pytest.mark.parametrize("n", [1024, 4096])
+@pytest.mark.parametrize("scale", [1e-30, 1e-6, 1.0, 1e6, 1e30])
+def test_split_mfma_any_magnitude(n, scale):
+    # Per-butterfly power-of-two scaling keeps both FP16 halves in range for
+    # data far outside FP16's own range.
+    from bfly import _lib
+    b = 9
+    x = (oracle.uniform_input(22, (b, 2 * n)).astype(np.float64) * scale).astype(np.float32)
+    got = _run_variant(x, n, b, -1, _lib.VARIANT_SPLIT_MFMA)
+    gate_c2c(got, x, n)
+
+
+@pytest.mark.parametrize("mode", sorted(oracle.NUMERICAL_MODES))
+def test_split_mfma_numerical_modes(mode):
+    from bfly import _lib
+    n, b = 2048, 3
+    x = oracle.numerical_input(n, b, mode, seed=1)
+    got = _run_variant(x, n, b, -1, _lib.VARIANT_SPLIT_MFMA)
+    ref = oracle.c2c_f64(x, n).astype(np.float32)
+    assert oracle.max_rel_scaled(got, ref) <= MAX_REL_SCALED
+
+
+def test_split_mfma_rejects_other_lengths():
+    import torch
+    from bfly import _lib
+    x = torch.zeros(2 * 512, device="cuda")
+    tab = _ops().table(0, 512, x.device)
+    assert _lib.hip().bfly_c2c_ex(x.data_ptr(), x.data_ptr(), 512, 1, -1, _lib.VARIANT_SPLIT_MFMA,
+                                  tab.data_ptr(), None, 0) == _lib.ERR_SHAPE
+
+
 @pytest.mark.parametrize("n", [16, 32, 64, 1024, 4096])
 def test_c2c_inverse_is_conjugate_transform(n):
     b = 5
@@ -95,6 +152,35 @@ def test_long_c2c_b64_seed2(n):
     x = oracle.uniform_input(2, (b, 2 * n))
     got = run_c2c(x, n)
     gate_c2c(got, x, n)
+
+
+@pytest.mark.parametrize("n", [8192, 16384])
+@pytest.mark.parametrize("b", [1, 5, 64])
+def test_single_workgroup_and_four_step_paths_agree(n, b):
+    # n = 8192 / 16384 run as one 512 / 1024-thread workgroup per transform
+    # by default; BFLY_VARIANT_FOUR_STEP forces the two-kernel path.  Both
+    # pass the FP64 gates on the same seeded input.
+    import torch
+    from bfly import _lib
+    ops = _ops()
+    x = oracle.uniform_input(2, (b, 2 * n))
+    t = torch.from_numpy(x).cuda()
+    tab = ops.table(_lib.KIND_C2C, n, t.device)
+    assert _lib.hip().bfly_workspace_bytes(_lib.KIND_C2C, n, b) == 0
+    work = torch.empty(n * b * 2, dtype=torch.float32, device="cuda")
+    outs = []
+    for variant, w in ((_lib.VARIANT_AUTO, None), (_lib.VARIANT_FOUR_STEP, work.data_ptr())):
+        y = torch.empty_like(t)
+        rc = _lib.hip().bfly_c2c_ex(t.data_ptr(), y.data_ptr(), n, b, -1, variant, tab.data_ptr(),
+                                    w, torch.cuda.current_stream().cuda_stream)
+        _lib.check(rc, "bfly_c2c_ex")
+        torch.cuda.synchronize()
+        outs.append(y.cpu().numpy())
+        gate_c2c(outs[-1], x, n)
+    # the four-step variant refuses to run without its workspace
+    y = torch.empty_like(t)
+    assert _lib.hip().bfly_c2c_ex(t.data_ptr(), y.data_ptr(), n, b, -1, _lib.VARIANT_FOUR_STEP,
+                                  tab.data_ptr(), None, 0) == _lib.ERR_SHAPE
 
 
 def test_long_c2c_matches_host_four_step_split():
