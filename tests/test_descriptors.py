@@ -1,0 +1,179 @@
+"""Long-FFT lowering descriptors (include/butterfly/descriptors.hpp).
+
+* Level 1 (legality and its reasons) agrees with the reference's own
+  query_lowering (include/butterfly/descriptors.hpp:409-486, compiled from the
+  reference sources into oracle/_ref) on every mutation of the reference's
+  default mapping, under the reference's hardware numbers.
+* Level 2 describes the gfx950 runtime: one launch on chip up to 16384
+  points, two launches with one device-memory boundary above; every count is
+  derived from the manifest and the modelled payload is launches x 2 x n x
+  batch x 8 bytes (the reference's accounting, tests/test_descriptors.py:
+  155-197 of the reference).
+"""
+import shutil
+import subprocess
+from pathlib import Path
+
+import pytest
+
+import oracle
+
+ROOT = Path(__file__).resolve().parents[1]
+
+# ArchitectureMapping enum values (same order as the reference)
+NONE, CELL, BLOCK, WORKER = 0, 1, 2, 3
+INTERLEAVED, TWIDDLED, NATURAL, HALF = 0, 1, 2, 3
+HOST, DEVICE_GM, ON_CHIP = 0, 1, 2
+SEPARATE, FUSED = 0, 1
+
+
+@pytest.fixture(scope="module")
+def harness(tmp_path_factory):
+    if not shutil.which("g++"):
+        pytest.skip("needs g++")
+    exe = tmp_path_factory.mktemp("desc") / "query_lowering_cases"
+    subprocess.run(["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", f"-I{ROOT / 'include'}",
+                    str(ROOT / "tests" / "cpp" / "query_lowering_cases.cpp"), "-o", str(exe)],
+                   check=True, capture_output=True, text=True)
+    return exe
+
+
+def _case(hw=0, n=8192, batch=1, precision=0, direction=0, n1=64, n2=128, partition=2, us=1, ts=2,
+          ud=256, td=1, role0=0, role1=1, residence=NONE, layout=TWIDDLED, home=DEVICE_GM,
+          impl=FUSED, pipeline=1, fold_d=0, plane_k=0, unit_ub=0, unit=0):
+    return [hw, n, batch, precision, direction, n1, n2, partition, us, ts, ud, td, role0, role1,
+            residence, layout, home, impl, pipeline, fold_d, plane_k, unit_ub, unit]
+
+
+def _run(exe, cases):
+    text = "\n".join(" ".join(str(int(v)) for v in c) for c in cases) + "\n"
+    out = subprocess.run([str(exe)], input=text, capture_output=True, text=True, check=True).stdout
+    res = []
+    for line in out.splitlines():
+        head, reason = line.split("#", 1)
+        f = head.split(" ")
+        res.append({"supported": int(f[0]), "abstract_feasible": int(f[1]), "launches": int(f[2]),
+                    "edges": int(f[3]), "host_assisted": int(f[4]), "on_chip": int(f[5]),
+                    "plan_ub": int(f[6]), "payload": int(f[7]), "kinds": f[8], "reason": reason})
+    assert len(res) == len(cases)
+    return res
+
+
+def _lds_len(n):
+    return n + n // 16
+
+
+def test_runtime_lowering_of_every_long_and_local_length(harness):
+    cases, want = [], []
+    for k in range(6, 17):
+        n = 1 << k
+        for b in (1, 64):
+            if n <= 16384:
+                ud = 256
+                cases.append(_case(n=n, batch=b, n1=n, n2=0, partition=1, ts=1, ud=ud, residence=BLOCK,
+                                   layout=NATURAL, home=ON_CHIP, impl=SEPARATE, unit=1))
+                lds = (256 // (n // 16)) * _lds_len(n) * 8 if n <= 4096 else _lds_len(n) * 8
+                want.append((1, 1, 0, 1, 2 * 8 * n * b, "row_fft", lds))
+            else:
+                n1 = 1 << (k // 2)
+                n2 = n // n1
+                cases.append(_case(n=n, batch=b, n1=n1, n2=n2))
+                lds = max((256 // (m // 16)) * (_lds_len(m) + 1) * 8 for m in (n1, n2))
+                want.append((1, 2, 1, 0, 2 * 2 * 8 * n * b, "row_fft|row_fft", lds))
+    for r, (sup, launches, edges, on_chip, payload, kinds, lds) in zip(_run(harness, cases), want):
+        assert r["supported"] == sup and r["abstract_feasible"] == 1, r
+        assert (r["launches"], r["edges"], r["on_chip"], r["host_assisted"]) == (launches, edges, on_chip, 0)
+        assert r["payload"] == payload == r["launches"] * 2 * (payload // (2 * r["launches"]))
+        assert r["kinds"] == kinds
+        assert r["plan_ub"] == lds and lds <= 160 * 1024
+
+
+def test_reference_chains_are_legal_but_not_lowered_on_gfx950(harness):
+    host, separate, fused = _run(harness, [_case(home=HOST, impl=SEPARATE), _case(impl=SEPARATE),
+                                           _case(impl=FUSED)])
+    assert host["supported"] == 0 and host["abstract_feasible"] == 1
+    assert host["reason"].startswith("abstract-feasible-but-not-lowered: boundary home host memory")
+    assert separate["supported"] == 0 and "fuses transposes and twiddle" in separate["reason"]
+    # the fused device chain: two launches here (the reference's manifest has five)
+    assert fused["supported"] == 1 and fused["launches"] == 2 and fused["edges"] == 1
+    assert fused["payload"] == 2 * 2 * 8192 * 8
+
+
+def test_one_stage_lowering_limits(harness):
+    res = _run(harness, [
+        _case(n=32768, n1=32768, n2=0, partition=1, ts=1, residence=BLOCK, layout=NATURAL, home=ON_CHIP, unit=1),
+        _case(n=8192, n1=8192, n2=0, partition=1, ts=1, residence=BLOCK, layout=NATURAL, home=ON_CHIP, unit=0),
+        _case(n=8192, n1=8192, n2=0, partition=1, ts=1, residence=NONE, layout=NATURAL, home=DEVICE_GM, unit=1),
+        _case(n=16384, n1=16384, n2=0, partition=1, ts=1, residence=BLOCK, layout=NATURAL, home=ON_CHIP,
+              unit=1, unit_ub=65536),
+    ])
+    assert res[0]["supported"] == 0 and "outside unit bfly_c2c_single range [4,14]" in res[0]["reason"]
+    assert res[1]["supported"] == 0 and "outside unit bfly_c2c_local range [4,12]" in res[1]["reason"]
+    assert res[2]["supported"] == 0 and res[2]["abstract_feasible"] == 1
+    assert res[3]["supported"] == 0 and res[3]["reason"].startswith("LDS overflow: kernel k_c2c_single")
+
+
+def _mutations():
+    """The reference's default host chain at 8192 = 64 x 128 and single-field
+    mutations of it; (our case, reference RefMapping fields)."""
+    base = dict(hw=1, n=8192, batch=2, n1=64, n2=128, partition=2, us=1, ts=2, ud=48, td=1, role0=0, role1=1,
+                residence=NONE, layout=TWIDDLED, home=HOST, impl=SEPARATE, pipeline=1, unit=2)
+    muts = [{}, {"precision": 1}, {"direction": 1}, {"direction": 2}, {"n": 100}, {"n": 32},
+            {"n1": 32, "n2": 256}, {"n1": 64, "n2": 64}, {"partition": 3}, {"partition": 0},
+            {"n1": 16, "n2": 512}, {"n1": 2, "n2": 4096}, {"n": 16384, "n1": 2, "n2": 8192},
+            {"us": 0}, {"ts": 0}, {"td": 0}, {"ud": 0}, {"ud": 49}, {"us": 49}, {"pipeline": 0},
+            {"role1": 2}, {"role0": -1}, {"layout": HALF}, {"residence": BLOCK}, {"residence": WORKER},
+            {"residence": CELL}, {"home": ON_CHIP}, {"batch": 1 << 30}, {"batch": 300000},
+            {"plane_k": 12}, {"plane_k": 64}, {"fold_d": 3, "plane_k": 8},
+            {"home": DEVICE_GM}, {"home": DEVICE_GM, "impl": FUSED}, {"us": 2}, {"ts": 1}, {"td": 2},
+            {"pipeline": 2}, {"ud": 8}]
+    out = []
+    for mu in muts:
+        d = dict(base)
+        d.update(mu)
+        out.append(d)
+    return out
+
+
+def _ref_lowering(d):
+    r = oracle.ref()
+    m = oracle.RefMapping()
+    m.n, m.batch, m.precision, m.direction = d["n"], d["batch"], d.get("precision", 0), d.get("direction", 0)
+    m.n1, m.n2, m.stage_partition = d["n1"], d["n2"], d["partition"]
+    m.stage_space, m.stage_time, m.data_space, m.data_time = d["us"], d["ts"], d["ud"], d["td"]
+    m.role0, m.role1, m.residence = d["role0"], d["role1"], d["residence"]
+    m.boundary_layout, m.boundary_home, m.boundary_impl = d["layout"], d["home"], d["impl"]
+    m.pipeline_buffers, m.fold_d, m.plane_k = d["pipeline"], d.get("fold_d", 0), d.get("plane_k", 0)
+    m.unit_ub_bytes = 0
+    out = oracle.RefLowering()
+    r.ref_query_lowering(m, None, None, None, out)
+    return {"supported": out.supported, "abstract_feasible": out.abstract_feasible,
+            "reason": out.reason.decode(), "launches": out.launches, "edges": out.gm_boundaries,
+            "payload": out.modeled_payload_gm}
+
+
+def test_level_one_reasons_match_the_reference(harness):
+    if oracle.ref() is None:
+        pytest.skip("oracle/_ref not built (reference sources absent)")
+    muts = _mutations()
+    ours = _run(harness, [_case(**{k: v for k, v in d.items()}) for d in muts])
+    checked = 0
+    for d, o in zip(muts, ours):
+        ref = _ref_lowering(d)
+        if not ref["abstract_feasible"] and not ref["reason"].startswith(("UB overflow", "illegal transpose")):
+            # rejected by a legality rule: same verdict, same words
+            assert o["abstract_feasible"] == 0 and o["supported"] == 0, (d, o, ref)
+            assert o["reason"] == ref["reason"], (d, o["reason"], ref["reason"])
+            checked += 1
+        elif ref["abstract_feasible"]:
+            assert o["abstract_feasible"] == 1, (d, o, ref)
+        # both models account payload the same way for whatever they lower
+        for x in (o, ref):
+            if x["supported"]:
+                assert x["payload"] == x["launches"] * 2 * d["n"] * d["batch"] * 8
+    assert checked >= 25
+    # the reference lowers its host chain and its separate device chain; this
+    # build lowers neither (the gfx950 chain is the fused two-launch one)
+    ref_host = _ref_lowering(muts[0])
+    assert ref_host["supported"] == 1 and ref_host["launches"] == 2
+    assert ours[0]["supported"] == 0 and ours[0]["abstract_feasible"] == 1
