@@ -57,7 +57,7 @@ SYMBOLS = (
     "bfly_rt_malloc", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
     "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
-    "bfly_c2r_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
+    "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
     "bfly_group_query", "bfly_group_destroy", "bfly_probe_copy", "bfly_probe_device",
 )
 
@@ -103,6 +103,7 @@ def _bind(lib):
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_r2c_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
         "bfly_c2r_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
+        "bfly_real_ex": (i32, [i32, vp, vp, u32, u32, u32, u32, i32, vp, vp]),
         "bfly_group_supported": (i32, [i32, u32, i32]),
         "bfly_group_create": (i32, [C.POINTER(GroupRequest), i32, C.POINTER(GroupConfig),
                                     C.POINTER(vp)]),

@@ -88,11 +88,20 @@ int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
     return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);
 }
 
+int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                 uint32_t stride, int variant, const float* table, void* stream) {
+    if (kind != BFLY_KIND_R2C && kind != BFLY_KIND_C2R) return BFLY_ERR_SHAPE;
+    const int sign = kind == BFLY_KIND_R2C ? -1 : 1;
+    if (int g = bfly_host::check_execute(kind, sign, in, out, n, batch, table, nullptr)) return g;
+    if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
+    if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_real_variant(n, batch);
+    if (variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM) return BFLY_ERR_SHAPE;
+    return launched(bfly_kern::real(kind, in, out, n, batch, bins, stride, variant, table, S(stream)));
+}
+
 int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                 uint32_t out_stride, const float* table, void* stream) {
-    if (int g = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return g;
-    if (!bfly_host::bins_ok(n, bins, out_stride)) return BFLY_ERR_SHAPE;
-    return launched(bfly_kern::real(BFLY_KIND_R2C, in, out, n, batch, bins, out_stride, table, S(stream)));
+    return bfly_real_ex(BFLY_KIND_R2C, in, out, n, batch, bins, out_stride, BFLY_VARIANT_AUTO, table, stream);
 }
 
 int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
@@ -102,9 +111,7 @@ int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const floa
 
 int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                 uint32_t in_stride, const float* table, void* stream) {
-    if (int g = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return g;
-    if (!bfly_host::bins_ok(n, bins, in_stride)) return BFLY_ERR_SHAPE;
-    return launched(bfly_kern::real(BFLY_KIND_C2R, in, out, n, batch, bins, in_stride, table, S(stream)));
+    return bfly_real_ex(BFLY_KIND_C2R, in, out, n, batch, bins, in_stride, BFLY_VARIANT_AUTO, table, stream);
 }
 
 int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,

@@ -117,7 +117,10 @@ __global__ __launch_bounds__(WG) void k_group(const GroupDesc* __restrict__ desc
         asm volatile("" : "+v"(tid));
         run_unit<WG, SC>(d, w.y, lds, tid, std::make_index_sequence<bfly_plan::kNumPlans>{});
         if (stamps) {
-            __threadfence();
+            // every store of this unit acknowledged, then one stamp per unit
+            // (a wait on this thread's memory counters, not a device-wide
+            // fence: that would write the L2 back and distort the timing)
+            __builtin_amdgcn_s_waitcnt(0);
             __syncthreads();
             if (threadIdx.x == 0) atomicMax(&stamps[1 + w.x], (unsigned long long)wall_clock64());
         }
@@ -131,8 +134,9 @@ __global__ __launch_bounds__(WG) void k_group(const GroupDesc* __restrict__ desc
             u += gridDim.x;
         }
     }
+    // A workgroup counts itself out only after its last fetch from ctr[0]
+    // returned, so the last one out knows every fetch is done.
     if (dynamic && threadIdx.x == 0) {
-        __threadfence();
         if (atomicAdd(&ctr[1], 1u) == gridDim.x - 1) {
             atomicExch(&ctr[0], 0u);
             atomicExch(&ctr[1], 0u);

@@ -171,7 +171,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
     __shared__ float2 lds[B * lds_len(N)];
     __shared__ float2 ltw[NLO + NHI];
     const int tid = threadIdx.x;
-    const int bl = tid / T, t = tid % T;
+    const int bl = tid / T, t0 = tid % T;
     for (int i = tid; i < NLO + NHI; i += bfly_plan::kBlock) ltw[i] = tw2g[i];
     __syncthreads();
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
@@ -184,9 +184,12 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
         for (int m = 0; m < P / R0; m++)
 #pragma unroll
             for (int r = 0; r < R0; r++)
-                cur[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+                cur[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t0 + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
     }
     for (; g < groups; g += gridDim.x) {
+        // opaque per-iteration thread index (see k_r2c_stream)
+        int t = t0;
+        asm volatile("" : "+v"(t));
         const uint32_t gn = g + gridDim.x;
         if (gn < groups) {
             const uint64_t row = (uint64_t)gn * B + bl;
@@ -228,6 +231,37 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
     r2c_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, kout, ostride, blockIdx.x, lds, threadIdx.x);
 }
 
+// R2C, streaming form: a grid sized to the resident capacity walks the
+// batch, loading the next group of rows while the current one computes.
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream(const float* __restrict__ in,
+                                                                  float* __restrict__ out,
+                                                                  const float2* __restrict__ tw,
+                                                                  const float2* __restrict__ wn,
+                                                                  uint32_t batch, uint32_t kout,
+                                                                  uint32_t ostride) {
+    constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
+    __shared__ float2 lds[B * lds_len(M)];
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    const int tid = threadIdx.x;
+    float2 cur[P], nxt[P];
+    uint32_t g = blockIdx.x;
+    if (g < groups) r2c_load<I, bfly_plan::kBlock>(cur, in, batch, g, tid);
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t gn = g + gridDim.x;
+        // an opaque per-iteration copy of the thread index keeps the compiler
+        // from hoisting every index-derived address out of the loop (their
+        // live ranges would double the register footprint)
+        int t2 = tid;
+        asm volatile("" : "+v"(t2));
+        if (gn < groups) r2c_load<I, bfly_plan::kBlock>(nxt, in, batch, gn, t2);
+        r2c_finish<I, bfly_plan::kBlock>(cur, out, tw, wn, batch, kout, ostride, g, lds, t2);
+        __syncthreads();  // the post-twiddle reads of `lds` end before the next FFT writes it
+#pragma unroll
+        for (int i = 0; i < P; i++) cur[i] = nxt[i];
+    }
+}
+
 // ------------------------------------------------------------------ C2R
 // Z'[k] = (X[k] + conj X[M-k]) + i W_n^{-k} (X[k] - conj X[M-k]), k < M,
 // with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
@@ -243,6 +277,33 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restri
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
     __shared__ float2 lds[B * lds_len(M)];
     c2r_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, scale, kin, istride, blockIdx.x, lds, threadIdx.x);
+}
+
+// C2R, streaming form (see k_r2c_stream).
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream(const float* __restrict__ in,
+                                                                  float* __restrict__ out,
+                                                                  const float2* __restrict__ tw,
+                                                                  const float2* __restrict__ wn,
+                                                                  uint32_t batch, float scale, uint32_t kin,
+                                                                  uint32_t istride) {
+    constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
+    __shared__ float2 lds[B * lds_len(M)];
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    const int tid = threadIdx.x;
+    float2 cur[P + 1], nxt[P + 1];
+    uint32_t g = blockIdx.x;
+    if (g < groups) c2r_load<I, bfly_plan::kBlock>(cur, in, batch, kin, istride, g, tid);
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t gn = g + gridDim.x;
+        int t2 = tid;  // opaque per iteration (see k_r2c_stream)
+        asm volatile("" : "+v"(t2));
+        if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
+        c2r_finish<I, bfly_plan::kBlock>(cur, out, tw, wn, batch, scale, g, lds, t2);
+        __syncthreads();  // the last exchange's reads end before the next staging writes
+#pragma unroll
+        for (int i = 0; i <= P; i++) cur[i] = nxt[i];
+    }
 }
 
 // ------------------------------------------------------------- long C2C
@@ -413,9 +474,26 @@ hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uin
 
 template <int I>
 hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, const float2* wn,
-                       uint32_t batch, uint32_t bins, uint32_t stride, hipStream_t s) {
+                       uint32_t batch, uint32_t bins, uint32_t stride, int variant, hipStream_t s) {
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
-    const dim3 grid((unsigned)((batch + B - 1) / B));
+    const uint64_t groups = (batch + B - 1) / B;
+    if (variant == BFLY_VARIANT_STREAM) {
+        const void* f = kind == 1 ? reinterpret_cast<const void*>(&k_r2c_stream<I>)
+                                  : reinterpret_cast<const void*>(&k_c2r_stream<I>);
+        static int cap[2] = {0, 0};
+        int& c = cap[kind == 1 ? 0 : 1];
+        if (c == 0) c = resident_blocks(f);
+        if (c <= 0) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
+        if (kind == 1)
+            hipLaunchKernelGGL((k_r2c_stream<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
+                               batch, bins, stride);
+        else
+            hipLaunchKernelGGL((k_c2r_stream<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
+                               batch, 1.0f / (float)(2 * PlanOf<I>::N), bins, stride);
+        return hipGetLastError();
+    }
+    const dim3 grid((unsigned)groups);
     if (kind == 1)
         hipLaunchKernelGGL((k_r2c<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch,
                            bins, stride);
@@ -502,20 +580,20 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
 }
 
 hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                uint32_t stride, const float* table, hipStream_t s) {
+                uint32_t stride, int variant, const float* table, hipStream_t s) {
     const float2* tw = reinterpret_cast<const float2*>(table);
     const bfly_host::TableLayout L = bfly_host::table_layout(kind, n);
     const float2* wn = tw + L.extra;
     switch (n / 2) {
-        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, bins, stride, s);
-        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, bins, stride, s);
+        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
         default: return hipErrorInvalidValue;
     }
 }

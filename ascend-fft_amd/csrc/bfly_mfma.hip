@@ -8,15 +8,18 @@
 // one half) by 16 such columns at once, so two instructions give all 32
 // output reals of 16 butterflies.
 //
-// Split FP16: every operand x is carried as hi = fp16(x) and
-// lo = fp16((x - hi) * 2^11), and the product is
-//     F x ~ Fhi xhi + 2^-11 (Fhi xlo + Flo xhi)
-// with FP32 accumulation (the lo*lo term, ~2^-22 relative, is dropped).
-// Each butterfly's inputs are first scaled by a power of two 2^-e that puts
-// their largest magnitude in [0.5, 1), so neither half over- nor underflows
-// whatever the data's range, and the outputs are scaled back by 2^e (both
-// exact).  Inter-pass twiddles stay FP32 on the vector ALUs, as do the
-// radix-4 / radix-8 remainder passes.
+// Split FP16: every operand x is carried as hi + lo, hi = x rounded toward
+// zero to FP16 precision and lo = fp16(x - hi), and the product is
+//     F x ~ Fhi xhi + Fhi xlo + Flo xhi
+// with FP32 accumulation (the lo*lo term, ~2^-22 relative, is dropped).  The
+// data enter scaled by one power of two per transform: its inputs are below
+// 2^e (e from the largest component: a wave reduction, plus an LDS exchange
+// when a transform spans several waves), every radix-16 pass grows a
+// component by less than 16 sqrt(2) < 2^5, so with P matrix passes the
+// scale 2^(11 - e - 5(P-1)) keeps every pass's inputs below 2^11 -- hi never
+// overflows FP16 whatever the data's range -- and the outputs are scaled
+// back by the same power of two (both exact).  Inter-pass twiddles stay FP32
+// on the vector ALUs, as do the radix-4 / radix-8 remainder passes.
 //
 // Lane layout of one MFMA pass: a wave owns 64 butterflies j = 64 w + 16 i
 // + c (i < 4, c = lane mod 16); lane group h = lane / 16 holds inputs
@@ -40,18 +43,19 @@ namespace {
 typedef _Float16 half8 __attribute__((ext_vector_type(8)));
 typedef float float4v __attribute__((ext_vector_type(4)));
 
-constexpr float kLoScale = 2048.0f;        // 2^11
-constexpr float kLoUnscale = 1.0f / 2048.0f;
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
 
-// hi / lo halves of x (|x| < 1 after the butterfly scale).
-__device__ __forceinline__ void split16(float x, _Float16& hi, _Float16& lo) {
-    hi = (_Float16)x;
-    lo = (_Float16)((x - (float)hi) * kLoScale);
-}
+// cos(2 pi m / 16), m = 0..15, indexed at run time by the lane's (q, r).
+__device__ const float kCos16[16] = {
+    1.0f, 0.92387953251128675613f, 0.70710678118654752440f, 0.38268343236508977173f,
+    0.0f, -0.38268343236508977173f, -0.70710678118654752440f, -0.92387953251128675613f,
+    -1.0f, -0.92387953251128675613f, -0.70710678118654752440f, -0.38268343236508977173f,
+    0.0f, 0.38268343236508977173f, 0.70710678118654752440f, 0.92387953251128675613f};
 
 // The A operands of one lane: rows 4-aligned block `rb` (outputs q = 8 rb ..
 // 8 rb + 7), row o = lane mod 16 (q = 8 rb + o / 2, Re for even o), columns
-// k = 8 h + kk (input r = 4 h + kk / 2, Re for even kk).
+// k = 8 h + kk (input r = 4 h + kk / 2, Re for even kk).  hi = fp16(a),
+// lo = fp16(a - hi).
 template <int DIR>
 __device__ __forceinline__ void dft16_operands(int lane, half8 (&ahi)[2], half8 (&alo)[2]) {
     const int o = lane & 15, h = lane >> 4;
@@ -64,53 +68,79 @@ __device__ __forceinline__ void dft16_operands(int lane, half8 (&ahi)[2], half8 
             const int r = 4 * h + (kk >> 1);
             const bool im_col = kk & 1;
             // F[q][r] = e^{DIR 2 pi i q r / 16} = cr + i ci
-            const float cr = cos16(q * r);
-            const float ci = DIR < 0 ? -sin16(q * r) : sin16(q * r);
+            const int m = (q * r) & 15;
+            const float cr = kCos16[m];
+            const float sn = kCos16[(m - 4) & 15];  // sin(2 pi m / 16)
+            const float ci = DIR < 0 ? -sn : sn;
             // Re y += cr Re x - ci Im x ; Im y += ci Re x + cr Im x
             const float a = im_row ? (im_col ? cr : ci) : (im_col ? -ci : cr);
-            _Float16 hi, lo;
-            split16(a, hi, lo);
+            const _Float16 hi = (_Float16)a;
             ahi[rb][kk] = hi;
-            alo[rb][kk] = lo;
+            alo[rb][kk] = (_Float16)(a - (float)hi);
         }
     }
 }
 
-// Radix-16 DFT of the lane's four butterflies: in v[4i + rr] = input
-// r = 4h + rr of butterfly i; out v[4i + s] = output q = 8 (s / 2) + 2h + s % 2.
+// Largest |component| of the lane's 16 values over the wave, as the exponent
+// e with max < 2^e (0 for all-zero data).
+__device__ __forceinline__ int wave_max_exponent(const float2 (&v)[16]) {
+    float m = 0.f;
+#pragma unroll
+    for (int i = 0; i < 16; i++) m = fmaxf(m, fmaxf(fabsf(v[i].x), fabsf(v[i].y)));
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) m = fmaxf(m, __shfl_xor(m, d));
+    int e = 0;
+    (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1)
+    return e;
+}
+
+// Radix-16 DFT of the lane's four butterflies (data already scaled): in
+// v[4i + rr] = input r = 4h + rr of butterfly i; out v[4i + s] = output
+// q = 8 (s / 2) + 2h + s % 2.
 __device__ __forceinline__ void dft16_mfma(float2 (&v)[16], const half8 (&ahi)[2], const half8 (&alo)[2]) {
+    // hi = x with its low 13 mantissa bits cleared (x rounded toward zero to
+    // FP16 precision, exact in FP16 for normal magnitudes), lo = x - hi exact
+    // in FP32, both packed two per dword
+    half8 bhi[4], blo[4];
 #pragma unroll
     for (int i = 0; i < 4; i++) {
-        float m = 0.f;
-#pragma unroll
-        for (int rr = 0; rr < 4; rr++) m = fmaxf(m, fmaxf(fabsf(v[4 * i + rr].x), fabsf(v[4 * i + rr].y)));
-        m = fmaxf(m, __shfl_xor(m, 16));
-        m = fmaxf(m, __shfl_xor(m, 32));
-        int e = 0;
-        (void)frexpf(m, &e);  // m = f 2^e, f in [0.5, 1); e = 0 for m = 0
-        const float down = ldexpf(1.0f, -e), up = ldexpf(1.0f, e);
-        half8 bhi, blo;
+        u32x4 ph, pl;
 #pragma unroll
         for (int rr = 0; rr < 4; rr++) {
-            _Float16 h0, l0, h1, l1;
-            split16(v[4 * i + rr].x * down, h0, l0);
-            split16(v[4 * i + rr].y * down, h1, l1);
-            bhi[2 * rr] = h0;
-            bhi[2 * rr + 1] = h1;
-            blo[2 * rr] = l0;
-            blo[2 * rr + 1] = l1;
+            const float x = v[4 * i + rr].x, y = v[4 * i + rr].y;
+            const float xh = __uint_as_float(__float_as_uint(x) & 0xFFFFE000u);
+            const float yh = __uint_as_float(__float_as_uint(y) & 0xFFFFE000u);
+            ph[rr] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(xh, yh));
+            pl[rr] = __builtin_bit_cast(uint32_t, __builtin_amdgcn_cvt_pkrtz(x - xh, y - yh));
         }
+        bhi[i] = __builtin_bit_cast(half8, ph);
+        blo[i] = __builtin_bit_cast(half8, pl);
+    }
+    // small terms first; eight independent accumulators per product
+    float4v acc[4][2];
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+            acc[i][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[rb], blo[i], float4v{0.f, 0.f, 0.f, 0.f},
+                                                                0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+            acc[i][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[rb], bhi[i], acc[i][rb], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int rb = 0; rb < 2; rb++)
+            acc[i][rb] = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[rb], bhi[i], acc[i][rb], 0, 0, 0);
+#pragma unroll
+    for (int i = 0; i < 4; i++)
 #pragma unroll
         for (int rb = 0; rb < 2; rb++) {
-            float4v acc = {0.f, 0.f, 0.f, 0.f};
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[rb], blo, acc, 0, 0, 0);
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(alo[rb], bhi, acc, 0, 0, 0);
-            acc *= kLoUnscale;
-            acc = __builtin_amdgcn_mfma_f32_16x16x32_f16(ahi[rb], bhi, acc, 0, 0, 0);
-            v[4 * i + 2 * rb] = make_float2(acc[0] * up, acc[1] * up);
-            v[4 * i + 2 * rb + 1] = make_float2(acc[2] * up, acc[3] * up);
+            v[4 * i + 2 * rb] = make_float2(acc[i][rb][0], acc[i][rb][1]);
+            v[4 * i + 2 * rb + 1] = make_float2(acc[i][rb][2], acc[i][rb][3]);
         }
-    }
 }
 
 // One radix-16 MFMA pass with Ns points already combined.  In: the lane's
@@ -141,11 +171,17 @@ __device__ __forceinline__ void pass16_mfma(float2 (&v)[16], float2* lds, const 
         for (int i = 0; i < 4; i++) {
             const int j = 64 * w + 16 * i + c;
             const int k = j & (Ns - 1);
-            const int base = (j - k) * 16 + k;
+            const int b2 = (j - k) * 16 + k + 2 * h * Ns;  // element q = 2h
 #pragma unroll
             for (int s = 0; s < 4; s++) {
-                const int q = 8 * (s >> 1) + 2 * h + (s & 1);
-                lds[lds_idx(base + q * Ns)] = v[4 * i + s];
+                // q - 2h = 8 (s / 2) + s % 2 is a compile-time offset
+                const int dq = (8 * (s >> 1) + (s & 1)) * Ns;
+                if constexpr (Ns >= 16) {
+                    // dq is a multiple of 16: pad(b2 + dq) = pad(b2) + dq * 17 / 16
+                    lds[lds_idx(b2) + dq + (dq >> 4)] = v[4 * i + s];
+                } else {
+                    lds[lds_idx(b2 + dq)] = v[4 * i + s];
+                }
             }
         }
     }
@@ -155,11 +191,16 @@ __device__ __forceinline__ void pass16_mfma(float2 (&v)[16], float2* lds, const 
 template <int N>
 __device__ __forceinline__ void read16_lds(float2 (&v)[16], const float2* lds, int w, int lane) {
     const int h = lane >> 4, c = lane & 15;
+    // N/16 is a multiple of 16, so every offset past the lane's first element
+    // pads by a compile-time amount
+    const int b = lds_idx(64 * w + c + 4 * h * (N / 16));
 #pragma unroll
     for (int i = 0; i < 4; i++)
 #pragma unroll
-        for (int rr = 0; rr < 4; rr++)
-            v[4 * i + rr] = lds[lds_idx(64 * w + 16 * i + c + (4 * h + rr) * (N / 16))];
+        for (int rr = 0; rr < 4; rr++) {
+            const int d = 16 * i + rr * (N / 16);  // multiple of 16
+            v[4 * i + rr] = lds[b + d + (d >> 4)];
+        }
 }
 
 // Plans [16, 16, R2]: two MFMA passes, then radix R2 -- on the matrix cores
@@ -175,7 +216,9 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
     static_assert(bfly_plan::kPlans[I].nrad == 3 && bfly_plan::kPlans[I].rad[0] == 16 &&
                       bfly_plan::kPlans[I].rad[1] == 16 && T % 64 == 0,
                   "plans [16, 16, R2] with whole waves per transform");
+    constexpr int WAVES = T / 64;  // waves per transform
     __shared__ float2 lds_all[B * lds_len(N)];
+    __shared__ int s_exp[B * WAVES];
     const int tid = threadIdx.x;
     const int bl = tid / T, t = tid % T;
     const int w = t >> 6, lane = t & 63, h = lane >> 4, c = lane & 15;
@@ -197,6 +240,20 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
 #pragma unroll
         for (int i = 0; i < 16; i++) v[i] = make_float2(0.f, 0.f);
     }
+    // one scale per transform: the largest input component over its waves
+    constexpr int PM = R2 == 16 ? 3 : 2;  // matrix passes
+    int et = wave_max_exponent(v);
+    if constexpr (WAVES > 1) {
+        if (lane == 0) s_exp[bl * WAVES + w] = et;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < WAVES; k++) et = max(et, s_exp[bl * WAVES + k]);
+    }
+    int sh = 11 - et - 5 * (PM - 1);
+    sh = sh > 126 ? 126 : (sh < -126 ? -126 : sh);
+    const float down = ldexpf(1.0f, sh), up = ldexpf(1.0f, -sh);
+#pragma unroll
+    for (int i = 0; i < 16; i++) v[i] = make_float2(v[i].x * down, v[i].y * down);
     // pass 0: Ns = 1
     pass16_mfma<N, 1, DIR, false>(v, lds, tw, w, lane, ahi, alo);
     __syncthreads();
@@ -217,7 +274,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
 #pragma unroll
             for (int s = 0; s < 4; s++) {
                 const int q = 8 * (s >> 1) + 2 * h + (s & 1);
-                st_io<true>(&dst[64 * w + 16 * i + c + q * (N / 16)], v[4 * i + s]);
+                st_io<true>(&dst[64 * w + 16 * i + c + q * (N / 16)],
+                            make_float2(v[4 * i + s].x * up, v[4 * i + s].y * up));
             }
     } else {
         // vector radix-R2 pass: thread t owns butterflies j = t + m T
@@ -239,7 +297,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
 #pragma unroll
         for (int m = 0; m < M; m++)
 #pragma unroll
-            for (int r = 0; r < R2; r++) st_io<true>(&dst[t + m * T + r * (N / R2)], v[m * R2 + r]);
+            for (int r = 0; r < R2; r++)
+                st_io<true>(&dst[t + m * T + r * (N / R2)], make_float2(v[m * R2 + r].x * up, v[m * R2 + r].y * up));
     }
 }
 

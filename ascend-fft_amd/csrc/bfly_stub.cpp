@@ -109,6 +109,17 @@ int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const floa
     if (int e = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return e;
     return launch();
 }
+int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
+                 uint32_t stride, int variant, const float* table, void*) {
+    if (kind != BFLY_KIND_R2C && kind != BFLY_KIND_C2R) return BFLY_ERR_SHAPE;
+    if (int e = bfly_host::check_execute(kind, kind == BFLY_KIND_R2C ? -1 : 1, in, out, n, batch, table,
+                                         nullptr))
+        return e;
+    if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
+    if (variant != BFLY_VARIANT_AUTO && variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM)
+        return BFLY_ERR_SHAPE;
+    return launch();
+}
 int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                 uint32_t stride, const float* table, void*) {
     if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;

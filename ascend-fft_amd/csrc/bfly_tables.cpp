@@ -116,9 +116,10 @@ int fill_table(int kind, uint32_t n, float* dst) {
 // dimension.
 bool split_mfma_ok(uint32_t n) { return n == 1024 || n == 2048 || n == 4096; }
 
-bool needs_work(int kind, uint32_t n, int variant) {
-    return kind == BFLY_KIND_C2C && n > (uint32_t)bfly_plan::kMaxLocalN &&
-           (n > (uint32_t)bfly_plan::kMaxSingleN || variant == BFLY_VARIANT_FOUR_STEP);
+bool needs_work(int kind, uint32_t n, uint32_t batch, int variant) {
+    if (kind != BFLY_KIND_C2C || n <= (uint32_t)bfly_plan::kMaxLocalN) return false;
+    if (variant == BFLY_VARIANT_AUTO) variant = auto_variant(n, batch);
+    return n > (uint32_t)bfly_plan::kMaxSingleN || variant == BFLY_VARIANT_FOUR_STEP;
 }
 
 static bool grid_ok(int kind, uint32_t n, uint32_t batch) {
@@ -147,7 +148,7 @@ int check_execute(int kind, int sign, const void* in, const void* out, uint32_t 
     if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
     if (!grid_ok(kind, n, batch)) return BFLY_ERR_SHAPE;
     if (!table) return BFLY_ERR_TABLE;
-    if (needs_work(kind, n, variant) && !work) return BFLY_ERR_SHAPE;
+    if (needs_work(kind, n, batch, variant) && !work) return BFLY_ERR_SHAPE;
     if (kind == BFLY_KIND_C2C && variant == BFLY_VARIANT_SPLIT_MFMA && !split_mfma_ok(n))
         return BFLY_ERR_SHAPE;
     return BFLY_OK;
@@ -157,11 +158,24 @@ int check_execute(int kind, int sign, const void* in, const void* out, uint32_t 
 // streaming form with LDS two-level twiddles wins from n = 512 once the batch
 // fills several waves of resident workgroups (n=4096 b=4096: 50.4 us vs
 // 62.2 us one-shot); below that the one-shot grid is as fast or faster.
+// One workgroup per transform at 8192 / 16384 beats the four-step chain once
+// the batch fills the compute units (8192 b=2048: 70.6 vs 111.4 us; 16384
+// b=1024: 82.0 vs 113.7 us); with few transforms the four-step chain's wider
+// grid wins (16384 b=64: 12.5 vs 17.0 us; 8192 b=64: a tie).
 int auto_variant(uint32_t n, uint32_t batch) {
     const uint64_t points = (uint64_t)n * batch;
     if (n >= 512 && n <= (uint32_t)bfly_plan::kMaxLocalN && points >= (1ull << 22))
         return BFLY_VARIANT_STREAM_LT;
+    if ((n == 8192 && batch < 64) || (n == 16384 && batch < 256)) return BFLY_VARIANT_FOUR_STEP;
     return BFLY_VARIANT_PLAIN;
+}
+
+// Real transforms: the streaming form once the batch fills several rounds of
+// resident workgroups (measured: R2C n=4096 b=4096 33.0 vs 37.8 us, C2R
+// n=256 b=65536 30.1 vs 34.5 us); at n = 8192 (one transform per workgroup)
+// the one-shot grid is as fast or faster.
+int auto_real_variant(uint32_t n, uint32_t batch) {
+    return (n <= 4096 && (uint64_t)n * batch >= (1ull << 22)) ? BFLY_VARIANT_STREAM : BFLY_VARIANT_PLAIN;
 }
 
 bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride) {
@@ -253,7 +267,7 @@ int bfly_table_fill(int kind, uint32_t n, float* host_table) {
 }
 
 size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch) {
-    if (!bfly_host::shape_ok(kind, n) || !bfly_host::needs_work(kind, n, BFLY_VARIANT_AUTO))
+    if (!bfly_host::shape_ok(kind, n) || !bfly_host::needs_work(kind, n, batch, BFLY_VARIANT_AUTO))
         return 0;
     return (size_t)n * (size_t)batch * 8u;
 }

@@ -37,9 +37,10 @@ int fill_table(int kind, uint32_t n, float* dst);
 int check_execute(int kind, int sign, const void* in, const void* out, uint32_t n,
                   uint32_t batch, const void* table, const void* work,
                   int variant = BFLY_VARIANT_AUTO);
-// True when a C2C of length n run as `variant` goes through the four-step
-// path and its n*batch complex workspace.
-bool needs_work(int kind, uint32_t n, int variant);
+// True when a C2C of (n, batch) run as `variant` (AUTO resolved by
+// auto_variant) goes through the four-step path and its n*batch complex
+// workspace.
+bool needs_work(int kind, uint32_t n, uint32_t batch, int variant);
 // Lengths the split-FP16 matrix-core variant runs (plans [16, 16, R]).
 bool split_mfma_ok(uint32_t n);
 
@@ -48,6 +49,8 @@ bool split_mfma_ok(uint32_t n);
 bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride);
 // The variant BFLY_VARIANT_AUTO resolves to for a C2C of (n, batch).
 int auto_variant(uint32_t n, uint32_t batch);
+// The variant BFLY_VARIANT_AUTO resolves to for a real transform of (n, batch).
+int auto_real_variant(uint32_t n, uint32_t batch);
 
 // ------------------------------------------------------- grouped launch
 // Whether a grouped launch with `wg` threads per workgroup takes (kind, n).

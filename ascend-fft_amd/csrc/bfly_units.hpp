@@ -112,21 +112,16 @@ __device__ __forceinline__ void c2c_unit(const float2* __restrict__ in, float2* 
 // Writes bins k < kout of each row at a row stride of ostride floats (the
 // full transform: kout = M+1, ostride = 2M+2; a mode-truncated one writes
 // only its first kout bins).
+// Load phase: the real row of transform g*B + bl read as M complex values
+// (zeros past the batch).
 template <int I, int WG, int SC = 0>
-__device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __restrict__ out,
-                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
-                                         uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
-                                         float2* lds, int tid) {
+__device__ __forceinline__ void r2c_load(float2 (&v)[bfly_plan::kPoints], const float* __restrict__ in,
+                                         uint32_t batch, uint32_t g, int tid) {
     using PL = PlanOf<I, SC>;
-    constexpr int M = PL::N, T = PL::T, B = WG / T;
-    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
-    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    constexpr int M = PL::N, T = PL::T, B = WG / T, R0 = PL::R0, P = PL::P;
     const int bl = tid / T, t = tid % T;
     const uint64_t tr = (uint64_t)g * B + bl;
-    const bool active = tr < batch;
-    float2* L = lds + bl * lds_len(M);
-    float2 v[P];
-    if (active) {
+    if (tr < batch) {
         const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)(2 * M));
 #pragma unroll
         for (int m = 0; m < P / R0; m++)
@@ -136,6 +131,22 @@ __device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __
 #pragma unroll
         for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
     }
+}
+
+// Compute phase: FFT_M, spectrum parked in LDS, split post-twiddle, store.
+// Ends with LDS reads still in flight in other threads: a caller that reuses
+// `lds` synchronises first.
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], float* __restrict__ out,
+                                           const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                           uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
+                                           float2* lds, int tid) {
+    using PL = PlanOf<I, SC>;
+    constexpr int M = PL::N, T = PL::T, B = WG / T, RL = PL::RL, P = PL::P;
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    float2* L = lds + bl * lds_len(M);
     fft_local<I, T, -1, SC>(v, L, tw, t);
     // Every LDS read of the last exchange completed before the barrier that
     // ends it, so the spectrum can be parked immediately.
@@ -144,7 +155,7 @@ __device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __
 #pragma unroll
         for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
     __syncthreads();
-    if (active) {
+    if (tr < batch) {
         float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
 #pragma unroll
         for (int i = 0; i <= P; i++) {
@@ -163,32 +174,58 @@ __device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __
     }
 }
 
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __restrict__ out,
+                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                         uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
+                                         float2* lds, int tid) {
+    float2 v[bfly_plan::kPoints];
+    r2c_load<I, WG, SC>(v, in, batch, g, tid);
+    r2c_finish<I, WG, SC>(v, out, tw, wn, batch, kout, ostride, g, lds, tid);
+}
+
 // ------------------------------------------------------------------ C2R
 // Z'[k] = (X[k] + conj X[M-k]) + i W_n^{-k} (X[k] - conj X[M-k]), k < M,
 // with Im X[0] = Im X[M] = 0; z = FFT^+_M(Z') / n; x[2j] = Re z, x[2j+1] = Im z.
 // Reads bins k < kin of each row at a row stride of istride floats; bins at
 // or above kin are zero (the full transform: kin = M+1, istride = 2M+2).
+// Load phase: bins 0..M of transform g*B + bl (k < kin, zero above), bin
+// k = t + i T in q[i] (i <= P; only t = 0 holds bin M).
 template <int I, int WG, int SC = 0>
-__device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __restrict__ out,
-                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
-                                         uint32_t batch, float scale, uint32_t kin, uint32_t istride,
-                                         uint32_t g, float2* lds, int tid) {
+__device__ __forceinline__ void c2r_load(float2 (&q)[bfly_plan::kPoints + 1], const float* __restrict__ in,
+                                         uint32_t batch, uint32_t kin, uint32_t istride, uint32_t g,
+                                         int tid) {
     using PL = PlanOf<I, SC>;
-    constexpr int M = PL::N, T = PL::T, B = WG / T;
-    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int M = PL::N, T = PL::T, B = WG / T, P = PL::P;
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    const bool active = tr < batch;
+    const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)istride);
+#pragma unroll
+    for (int i = 0; i <= P; i++) {
+        const int k = t + i * T;
+        q[i] = (active && k <= M && k < (int)kin) ? ld_io<(T >= 16)>(&src[k]) : make_float2(0.f, 0.f);
+    }
+}
+
+// Compute phase: bins staged in LDS, Hermitian pre-twiddle, inverse FFT_M,
+// reals scaled and stored.  Starts by writing `lds`: a caller that reuses it
+// synchronises first.
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void c2r_finish(float2 (&q)[bfly_plan::kPoints + 1], float* __restrict__ out,
+                                           const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                           uint32_t batch, float scale, uint32_t g, float2* lds, int tid) {
+    using PL = PlanOf<I, SC>;
+    constexpr int M = PL::N, T = PL::T, B = WG / T, R0 = PL::R0, RL = PL::RL, P = PL::P;
     static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
     static_assert(WG % T == 0, "a work unit holds whole transforms");
     const int bl = tid / T, t = tid % T;
     const uint64_t tr = (uint64_t)g * B + bl;
-    const bool active = tr < batch;
     float2* L = lds + bl * lds_len(M);
-    if (active) {
-        const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)istride);
 #pragma unroll
-        for (int i = 0; i <= P; i++) {
-            const int k = t + i * T;
-            if (k <= M) L[k] = k < (int)kin ? ld_io<(T >= 16)>(&src[k]) : make_float2(0.f, 0.f);
-        }
+    for (int i = 0; i <= P; i++) {
+        const int k = t + i * T;
+        if (k <= M) L[k] = q[i];
     }
     __syncthreads();
     float2 v[P];
@@ -206,7 +243,7 @@ __device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __
         }
     __syncthreads();
     fft_local<I, T, 1, SC>(v, L, tw, t);
-    if (!active) return;
+    if (tr >= batch) return;
     float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M));
 #pragma unroll
     for (int m = 0; m < P / RL; m++)
@@ -215,6 +252,16 @@ __device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __
             const float2 z = v[m * RL + r];
             st_io<(T >= 16)>(&dst[t + m * T + r * (M / RL)], make_float2(z.x * scale, z.y * scale));
         }
+}
+
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __restrict__ out,
+                                         const float2* __restrict__ tw, const float2* __restrict__ wn,
+                                         uint32_t batch, float scale, uint32_t kin, uint32_t istride,
+                                         uint32_t g, float2* lds, int tid) {
+    float2 q[bfly_plan::kPoints + 1];
+    c2r_load<I, WG, SC>(q, in, batch, kin, istride, g, tid);
+    c2r_finish<I, WG, SC>(q, out, tw, wn, batch, scale, g, lds, tid);
 }
 
 }  // namespace bfly_dev

@@ -86,10 +86,11 @@ BFLY_API size_t bfly_table_floats(int kind, uint32_t n);
 BFLY_API int bfly_table_fill(int kind, uint32_t n, float* host_table);
 
 /* Device workspace (bytes) an execute call with BFLY_VARIANT_AUTO needs; 0
- * for every length that runs in one kernel (C2C up to 16384, all real
- * transforms).  C2C above 16384 runs four-step and needs n*batch complex
- * values for the intermediate; so does BFLY_VARIANT_FOUR_STEP at 8192 and
- * 16384. */
+ * for every shape that runs in one kernel (C2C up to 4096, 8192 with
+ * batch >= 64, 16384 with batch >= 256; all real transforms).  The four-step
+ * path (C2C above 16384, the smaller batches of 8192 / 16384, and
+ * BFLY_VARIANT_FOUR_STEP) needs n*batch complex values for the
+ * intermediate. */
 BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
 
 /* --------------------------------------------------------------- execute
@@ -168,6 +169,16 @@ BFLY_API int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch
                          uint32_t out_stride, const float* table, void* stream);
 BFLY_API int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                          uint32_t in_stride, const float* table, void* stream);
+
+/* Real transforms with an explicit kernel variant (kind R2C or C2R; bins and
+ * stride as for bfly_r2c_ex / bfly_c2r_ex): BFLY_VARIANT_PLAIN = one-shot
+ * grid, BFLY_VARIANT_STREAM = resident grid that loads the next rows while
+ * computing the current ones (same arithmetic, same bits), BFLY_VARIANT_AUTO
+ * = the library's choice; other ids give BFLY_ERR_SHAPE.  Same reference
+ * interfaces as bfly_r2c / bfly_c2r. */
+BFLY_API int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
+                          uint32_t bins, uint32_t stride, int variant, const float* table,
+                          void* stream);
 
 /* ------------------------------------------------------- grouped launch
  * Many independent transforms of different kinds, lengths and batch sizes

@@ -102,8 +102,13 @@ def test_long_table_layout(n):
     single = {8192: [16, 16, 16, 2], 16384: [16, 16, 16, 4]}.get(n)
     tail = _stage_table(single).astype(np.complex64) if single else np.zeros(0, np.complex64)
     assert np.array_equal(c[o + n:], tail)
-    # only the four-step lengths need the n*batch workspace on the default path
-    assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 64) == (n * 64 * 8 if n > 16384 else 0)
+    # the default path needs the n*batch workspace only where it runs four-step:
+    # above 16384, and below the batch that fills the GPU one transform per
+    # workgroup (8192: 64, 16384: 256)
+    assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 64) == (0 if n == 8192 else n * 64 * 8)
+    if n <= 16384:
+        assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 256) == 0
+        assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 63) == n * 63 * 8
 
 
 @pytest.mark.parametrize("kind", [bfly.KIND_R2C, bfly.KIND_C2R])
@@ -145,15 +150,17 @@ def test_execute_guards_through_stub():
     assert s.bfly_c2c(p, p, 4096, 1, -1, None, None, None) == _lib.ERR_TABLE
     assert s.bfly_c2c(p, p, 32768, 1, -1, p, None, None) == bfly.ERR_SHAPE  # four-step needs work
     assert s.bfly_c2c(p, p, 32768, 1, -1, p, p, None) == 0
-    assert s.bfly_c2c(p, p, 8192, 1, -1, p, None, None) == 0  # one workgroup per transform
-    assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_FOUR_STEP, p, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(p, p, 8192, 64, -1, p, None, None) == 0  # one workgroup per transform
+    assert s.bfly_c2c(p, p, 8192, 1, -1, p, None, None) == bfly.ERR_SHAPE  # small batch: four-step
+    assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_PLAIN, p, None, None) == 0
+    assert s.bfly_c2c_ex(p, p, 8192, 64, -1, _lib.VARIANT_FOUR_STEP, p, None, None) == bfly.ERR_SHAPE
     assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_LAST + 1, p, None, None) == bfly.ERR_SHAPE
     assert s.bfly_r2c(p, p, 16, 1, p, None) == bfly.ERR_SHAPE
     assert s.bfly_c2r(p, p, 16384, 1, p, None) == bfly.ERR_SHAPE
     assert s.bfly_c2c(p, p, 4096, 0x7FFFFFFF, -1, p, None, None) == 0  # grid fits
     cnt = (C.c_int * 8)()
     s.bfly_stub_counters(cnt)
-    assert cnt[3] == 4  # launches counted only for accepted calls
+    assert cnt[3] == 5  # launches counted only for accepted calls
 
 
 def test_python_modules_import_and_variant_ids_match_header():

@@ -136,6 +136,49 @@ def test_split_mfma_rejects_other_lengths():
                                   tab.data_ptr(), None, 0) == _lib.ERR_SHAPE
 
 
+@pytest.mark.parametrize("n", [32, 256, 1024, 4096, 8192])
+def test_real_streaming_variant_equals_one_shot(n):
+    # bfly_real_ex STREAM runs the same arithmetic as PLAIN on a resident
+    # grid: outputs equal bit for bit (ragged and multi-round batches,
+    # full and mode-truncated spectra), and both pass the FP64 gate.
+    import torch
+    from bfly import _lib
+    lib = _lib.hip()
+    s = torch.cuda.current_stream().cuda_stream
+    for b, bins in ((1, n // 2 + 1), (333, n // 2 + 1), (2000, 7)):
+        x = oracle.uniform_input(31, (b, n))
+        xt = torch.from_numpy(x).cuda()
+        tab = _ops().table(1, n, xt.device)
+        outs = []
+        for v in (_lib.VARIANT_PLAIN, _lib.VARIANT_STREAM):
+            y = torch.full((b, 2 * bins), float("nan"), device="cuda")
+            assert lib.bfly_real_ex(1, xt.data_ptr(), y.data_ptr(), n, b, bins, 2 * bins, v,
+                                    tab.data_ptr(), s) == 0
+            outs.append(y)
+        torch.cuda.synchronize()
+        assert torch.equal(outs[0], outs[1]), (n, b, bins)
+        ref = oracle.r2c_f64(x, n)[:, :2 * bins]
+        assert oracle.rel_l2_rows(outs[1].cpu().numpy(), np.ascontiguousarray(ref)) <= REL_L2
+        h = torch.zeros(b, n + 2, device="cuda")
+        h[:, :2 * bins] = outs[1]
+        h[:, 1] = 0
+        if bins == n // 2 + 1:
+            h[:, n + 1] = 0
+        tab2 = _ops().table(2, n, xt.device)
+        backs = []
+        for v in (_lib.VARIANT_PLAIN, _lib.VARIANT_STREAM):
+            z = torch.full((b, n), float("nan"), device="cuda")
+            assert lib.bfly_real_ex(2, h.data_ptr(), z.data_ptr(), n, b, bins, n + 2, v,
+                                    tab2.data_ptr(), s) == 0
+            backs.append(z)
+        torch.cuda.synchronize()
+        assert torch.equal(backs[0], backs[1]), (n, b, bins)
+        assert oracle.rel_l2_rows(backs[1].cpu().numpy(), oracle.c2r_f64(h.cpu().numpy(), n)) <= REL_L2
+    # ids other than AUTO / PLAIN / STREAM are refused
+    assert lib.bfly_real_ex(1, xt.data_ptr(), xt.data_ptr(), n, 1, n // 2 + 1, n + 2,
+                            _lib.VARIANT_STREAM_LT, tab.data_ptr(), s) == _lib.ERR_SHAPE
+
+
 @pytest.mark.parametrize("n", [16, 32, 64, 1024, 4096])
 def test_c2c_inverse_is_conjugate_transform(n):
     b = 5
@@ -157,19 +200,20 @@ def test_long_c2c_b64_seed2(n):
 @pytest.mark.parametrize("n", [8192, 16384])
 @pytest.mark.parametrize("b", [1, 5, 64])
 def test_single_workgroup_and_four_step_paths_agree(n, b):
-    # n = 8192 / 16384 run as one 512 / 1024-thread workgroup per transform
-    # by default; BFLY_VARIANT_FOUR_STEP forces the two-kernel path.  Both
-    # pass the FP64 gates on the same seeded input.
+    # n = 8192 / 16384 as one 512 / 1024-thread workgroup per transform
+    # (BFLY_VARIANT_PLAIN, no workspace) and as the two-kernel four-step path
+    # (BFLY_VARIANT_FOUR_STEP); AUTO picks per batch.  All pass the FP64 gates
+    # on the same seeded input.
     import torch
     from bfly import _lib
     ops = _ops()
     x = oracle.uniform_input(2, (b, 2 * n))
     t = torch.from_numpy(x).cuda()
     tab = ops.table(_lib.KIND_C2C, n, t.device)
-    assert _lib.hip().bfly_workspace_bytes(_lib.KIND_C2C, n, b) == 0
     work = torch.empty(n * b * 2, dtype=torch.float32, device="cuda")
     outs = []
-    for variant, w in ((_lib.VARIANT_AUTO, None), (_lib.VARIANT_FOUR_STEP, work.data_ptr())):
+    for variant, w in ((_lib.VARIANT_PLAIN, None), (_lib.VARIANT_FOUR_STEP, work.data_ptr()),
+                       (_lib.VARIANT_AUTO, work.data_ptr())):
         y = torch.empty_like(t)
         rc = _lib.hip().bfly_c2c_ex(t.data_ptr(), y.data_ptr(), n, b, -1, variant, tab.data_ptr(),
                                     w, torch.cuda.current_stream().cuda_stream)
