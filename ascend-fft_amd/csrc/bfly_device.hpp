@@ -16,6 +16,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include "bfly_plans.hpp"
+
 namespace bfly_dev {
 
 // ------------------------------------------------------- streaming access
@@ -400,17 +402,24 @@ __device__ __forceinline__ float2 tw2(const float2* tlo, const float2* thi, int 
     }
 }
 
+// `t1` holds the second pass's own table (W_{Ns R}^{r k} at [k R + r], Ns =
+// the first radix) in LDS: one read per twiddle there; later passes combine
+// the two-level tables.
 template <int N, int P, int T, int DIR, int LX, int Ns, int R, int... Rest>
 __device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb, const float2* tlo,
-                                            const float2* thi, int t) {
+                                            const float2* thi, const float2* t1, int t) {
     constexpr int M = P / R;
     if constexpr (Ns > 1) {
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int k = (t + m * T) & (Ns - 1);
 #pragma unroll
-            for (int r = 1; r < R; r++)
-                v[m * R + r] = cmul_tw<DIR>(v[m * R + r], tw2<N>(tlo, thi, r * k * (N / (Ns * R))));
+            for (int r = 1; r < R; r++) {
+                if constexpr (Ns == bfly_plan::kPoints)
+                    v[m * R + r] = cmul_tw<DIR>(v[m * R + r], t1[k * R + r]);
+                else
+                    v[m * R + r] = cmul_tw<DIR>(v[m * R + r], tw2<N>(tlo, thi, r * k * (N / (Ns * R))));
+            }
         }
     }
 #pragma unroll
@@ -434,7 +443,7 @@ __device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb,
             for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lx_rat<LX, N / R2>(rbase, r)];
         }
         __syncthreads();
-        stockham_lt<N, P, T, DIR, LX, Ns * R, Rest...>(v, lds, rb, tlo, thi, t);
+        stockham_lt<N, P, T, DIR, LX, Ns * R, Rest...>(v, lds, rb, tlo, thi, t1, t);
     }
 }
 

@@ -56,7 +56,7 @@ __global__ __launch_bounds__(PlanOf<I>::T) void k_c2c_single(
 // of its next transform before computing the current one, so HBM reads stay
 // in flight through the LDS exchanges and the register butterflies.
 template <int I, int DIR, int LX>
-__global__ __launch_bounds__(bfly_plan::kBlock) __attribute__((amdgpu_waves_per_eu(3))) void k_c2c_stream(
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
     uint32_t batch) {
     using PL = PlanOf<I>;
@@ -65,10 +65,10 @@ __global__ __launch_bounds__(bfly_plan::kBlock) __attribute__((amdgpu_waves_per_
     constexpr int TWR = plan_twiddle_regs<I>(std::make_index_sequence<PL::NRAD>{});
     __shared__ float2 lds[B * lx_len<LX>(N)];
     const int tid = threadIdx.x;
-    const int bl = tid / T, t = tid % T;
+    const int bl = tid / T, t0 = tid % T;
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
     float2 tr[TWR > 0 ? TWR : 1];
-    load_plan_twiddles<I, T>(tr, tw, t, std::make_index_sequence<PL::NRAD>{});
+    load_plan_twiddles<I, T>(tr, tw, t0, std::make_index_sequence<PL::NRAD>{});
     float2 cur[P], nxt[P];
     uint32_t g = blockIdx.x;
     if (g < groups) {
@@ -78,9 +78,11 @@ __global__ __launch_bounds__(bfly_plan::kBlock) __attribute__((amdgpu_waves_per_
         for (int m = 0; m < P / R0; m++)
 #pragma unroll
             for (int r = 0; r < R0; r++)
-                cur[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
+                cur[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t0 + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
     }
     for (; g < groups; g += gridDim.x) {
+        int t = t0;  // opaque per iteration (see k_r2c_stream)
+        asm volatile("" : "+v"(t));
         const uint32_t gn = g + gridDim.x;
         if (gn < groups) {
             const uint64_t row = (uint64_t)gn * B + bl;
@@ -161,18 +163,22 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream2(
 // flight per CU.  `tw2g` points at [A (64) | B (N/64)] in the table.
 template <int I, int DIR>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
-    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw2g,
-    uint32_t batch) {
+    const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
+    const float2* __restrict__ tw2g, uint32_t batch) {
     using PL = PlanOf<I>;
     constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
     constexpr int NLO = N < TW2_LO ? N : TW2_LO;
     constexpr int NHI = N / NLO;
     __shared__ float2 lds[B * lds_len(N)];
-    __shared__ float2 ltw[NLO + NHI];
+    // the second pass's table (16 R1 entries, the first of the stage table)
+    constexpr int NT1 = PL::NRAD > 1 ? bfly_plan::kPoints * kPlans[I].rad[PL::NRAD > 1 ? 1 : 0] : 1;
+    __shared__ float2 ltw[NLO + NHI + NT1];
     const int tid = threadIdx.x;
     const int bl = tid / T, t0 = tid % T;
     for (int i = tid; i < NLO + NHI; i += bfly_plan::kBlock) ltw[i] = tw2g[i];
+    if constexpr (PL::NRAD > 1)
+        for (int i = tid; i < NT1; i += bfly_plan::kBlock) ltw[NLO + NHI + i] = tw[i];
     __syncthreads();
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
     float2 cur[P], nxt[P];
@@ -200,7 +206,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
                 for (int r = 0; r < R0; r++)
                     nxt[m * R0 + r] = row < batch ? ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]) : make_float2(0.f, 0.f);
         }
-        run_plan_lt<I, T, DIR, LX_PAD>(cur, lds + bl * lds_len(N), 0, ltw, ltw + NLO, t,
+        run_plan_lt<I, T, DIR, LX_PAD>(cur, lds + bl * lds_len(N), 0, ltw, ltw + NLO, ltw + NLO + NHI, t,
                                        std::make_index_sequence<PL::NRAD>{});
         const uint64_t row = (uint64_t)g * B + bl;
         if (row < batch) {
@@ -427,15 +433,16 @@ hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint3
 }
 
 template <int I, int DIR>
-hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw2g, uint32_t batch,
-                         hipStream_t s) {
+hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw, const float2* tw2g,
+                         uint32_t batch, hipStream_t s) {
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
     const uint64_t groups = (batch + B - 1) / B;
     static int cap = 0;
     if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>));
     if (cap <= 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)(groups < (uint64_t)cap ? groups : (uint64_t)cap));
-    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw2g, batch);
+    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, tw2g,
+                       batch);
     return hipGetLastError();
 }
 
@@ -460,8 +467,8 @@ hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uin
     }
     if (variant == BFLY_VARIANT_STREAM_LT) {
         const float2* tw2g = tw + bfly_host::table_layout(0, PlanOf<I>::N).extra;
-        return sign < 0 ? launch_c2c_lt<I, -1>(in, out, tw2g, batch, s)
-                        : launch_c2c_lt<I, 1>(in, out, tw2g, batch, s);
+        return sign < 0 ? launch_c2c_lt<I, -1>(in, out, tw, tw2g, batch, s)
+                        : launch_c2c_lt<I, 1>(in, out, tw, tw2g, batch, s);
     }
     const bool stream = variant == BFLY_VARIANT_STREAM || variant == BFLY_VARIANT_STREAM_SWZ;
     const bool swz = variant == BFLY_VARIANT_PLAIN_SWZ || variant == BFLY_VARIANT_STREAM_SWZ;

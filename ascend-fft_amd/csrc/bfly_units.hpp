@@ -44,9 +44,10 @@ __device__ __forceinline__ void run_plan_rt(float2 (&v)[bfly_plan::kPoints], flo
 
 template <int I, int T, int DIR, int LX, size_t... S>
 __device__ __forceinline__ void run_plan_lt(float2 (&v)[bfly_plan::kPoints], float2* lds, int rb,
-                                            const float2* tlo, const float2* thi, int t,
+                                            const float2* tlo, const float2* thi, const float2* t1, int t,
                                             std::index_sequence<S...>) {
-    stockham_lt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tlo, thi, t);
+    stockham_lt<kPlans[I].n, bfly_plan::kPoints, T, DIR, LX, 1, kPlans[I].rad[S]...>(v, lds, rb, tlo, thi,
+                                                                                     t1, t);
 }
 
 template <int I, int T, size_t... S>

@@ -52,6 +52,9 @@ def parse():
                     help="untimed steps before the warmup steps (GPU clock ramp)")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--only", default=None,
+                    choices=["grouped", "r2c", "c2r", "mfma", "long65536", "single16384"],
+                    help="run one secondary workload loop instead of the headline (profiling)")
     ap.add_argument("--no-patterns", action="store_true",
                     help="skip the grouped uneven-batch measurement")
     ap.add_argument("--probe-copy", action="store_true",
@@ -210,12 +213,75 @@ def spectral_pattern(reps: int = 10):
     }
 
 
+def workload_loop(name: str, steps: int, warmup: int):
+    """One secondary workload run `steps` times after `warmup` untimed runs,
+    for kernel traces and counter passes of its kernels (--only NAME).
+    Prints one JSON line with the mean ms per step (HIP events)."""
+    import numpy as np
+    import torch
+    from bfly import _lib, group, ops
+
+    lib = _lib.hip()
+    dev = torch.device("cuda:0")
+    sid = lambda: torch.cuda.current_stream().cuda_stream  # noqa: E731
+    g = torch.Generator(device=dev)
+    g.manual_seed(5)
+
+    def rand(*shape):
+        return torch.rand(*shape, generator=g, device=dev) * 2 - 1
+
+    if name == "grouped":
+        reqs = group.make_requests(group.uneven_specs(), seed=3)
+        grp = group.Group(reqs, group.Config(dynamic=1))
+        graph = group.capture(grp.launch)
+        fn, nbytes = graph.replay, grp.algorithmic_bytes()
+    elif name in ("r2c", "c2r"):
+        n, b = 4096, 4096
+        x, h = rand(b, n), rand(b, n + 2)
+        h[:, 1] = 0
+        h[:, n + 1] = 0
+        kind = 1 if name == "r2c" else 2
+        tab = ops.table(kind, n, dev)
+        src, dst = (x, torch.empty_like(h)) if kind == 1 else (h, torch.empty_like(x))
+        fn = lambda: _lib.check(lib.bfly_real_ex(kind, src.data_ptr(), dst.data_ptr(), n, b, n // 2 + 1,  # noqa: E731
+                                                 n + 2, _lib.VARIANT_AUTO, tab.data_ptr(), sid()), name)
+        nbytes = (4 * n + 4 * (n + 2)) * b
+    elif name in ("mfma", "long65536", "single16384"):
+        n, b, v = {"mfma": (4096, 4096, _lib.VARIANT_SPLIT_MFMA), "long65536": (65536, 256, _lib.VARIANT_AUTO),
+                   "single16384": (16384, 1024, _lib.VARIANT_AUTO)}[name]
+        x = rand(b, 2 * n)
+        y = torch.empty_like(x)
+        tab = ops.table(0, n, dev)
+        work = ops.workspace(0, n, b, dev)
+        wp = work.data_ptr() if work is not None else None
+        fn = lambda: _lib.check(lib.bfly_c2c_ex(x.data_ptr(), y.data_ptr(), n, b, -1, v, tab.data_ptr(),  # noqa: E731
+                                                wp, sid()), name)
+        nbytes = 16 * n * b
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(steps):
+        fn()
+    e1.record()
+    e1.synchronize()
+    ms = e0.elapsed_time(e1) / steps
+    print(json.dumps({"workload": name, "steps": steps, "ms_per_step": round(ms, 5),
+                      "algorithmic_bytes": int(nbytes), "GB_per_s": round(nbytes / (ms * 1e-3) / 1e9, 1)}))
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on exactly one GPU; --gpus N>1 is not "
                                    "supported (batch rows would shard with no communication)"}))
         sys.exit(2)
+    if args.only:
+        workload_loop(args.only, args.steps, args.warmup)
+        return
     import numpy as np
     import torch
     from bfly import ops, tune
