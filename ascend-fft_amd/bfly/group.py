@@ -251,7 +251,7 @@ def per_request_latencies_us(reqs: list[Request], variant: int = _lib.VARIANT_PL
 DEFAULT_SPACE = {
     "workgroup": (512, 1024),
     "schedule": (0, 1),
-    "order": (0, 1, 2),
+    "order": (0, 1, 2, 3),
     "dynamic": (0, 1),
     "grid": (0,),
 }

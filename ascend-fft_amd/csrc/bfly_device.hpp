@@ -402,9 +402,10 @@ __device__ __forceinline__ float2 tw2(const float2* tlo, const float2* thi, int 
     }
 }
 
-// `t1` holds the second pass's own table (W_{Ns R}^{r k} at [k R + r], Ns =
-// the first radix) in LDS: one read per twiddle there; later passes combine
-// the two-level tables.
+// `t1` holds the second pass's own table in LDS, transposed to [r Ns + k]
+// (Ns = the first radix) so that the lanes of a wave, whose k are
+// consecutive, read consecutive banks: one read per twiddle there; later
+// passes combine the two-level tables.
 template <int N, int P, int T, int DIR, int LX, int Ns, int R, int... Rest>
 __device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb, const float2* tlo,
                                             const float2* thi, const float2* t1, int t) {
@@ -416,7 +417,7 @@ __device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb,
 #pragma unroll
             for (int r = 1; r < R; r++) {
                 if constexpr (Ns == bfly_plan::kPoints)
-                    v[m * R + r] = cmul_tw<DIR>(v[m * R + r], t1[k * R + r]);
+                    v[m * R + r] = cmul_tw<DIR>(v[m * R + r], t1[r * Ns + k]);
                 else
                     v[m * R + r] = cmul_tw<DIR>(v[m * R + r], tw2<N>(tlo, thi, r * k * (N / (Ns * R))));
             }

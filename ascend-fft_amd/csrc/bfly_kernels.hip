@@ -171,14 +171,17 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
     constexpr int NLO = N < TW2_LO ? N : TW2_LO;
     constexpr int NHI = N / NLO;
     __shared__ float2 lds[B * lds_len(N)];
-    // the second pass's table (16 R1 entries, the first of the stage table)
-    constexpr int NT1 = PL::NRAD > 1 ? bfly_plan::kPoints * kPlans[I].rad[PL::NRAD > 1 ? 1 : 0] : 1;
+    // the second pass's table (16 R1 entries, the first of the stage table,
+    // stored [k R1 + r]), transposed into LDS as [r 16 + k]
+    constexpr int R1 = kPlans[I].rad[PL::NRAD > 1 ? 1 : 0];
+    constexpr int NT1 = PL::NRAD > 1 ? bfly_plan::kPoints * R1 : 1;
     __shared__ float2 ltw[NLO + NHI + NT1];
     const int tid = threadIdx.x;
     const int bl = tid / T, t0 = tid % T;
     for (int i = tid; i < NLO + NHI; i += bfly_plan::kBlock) ltw[i] = tw2g[i];
     if constexpr (PL::NRAD > 1)
-        for (int i = tid; i < NT1; i += bfly_plan::kBlock) ltw[NLO + NHI + i] = tw[i];
+        for (int i = tid; i < NT1; i += bfly_plan::kBlock)
+            ltw[NLO + NHI + (i % R1) * bfly_plan::kPoints + i / R1] = tw[i];
     __syncthreads();
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
     float2 cur[P], nxt[P];

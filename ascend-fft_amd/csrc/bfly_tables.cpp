@@ -203,7 +203,7 @@ int group_plan(const bfly_group_request* reqs, int nreq, const bfly_group_config
     if (cfg) c = *cfg;
     if (c.workgroup == 0) c.workgroup = 512;
     if ((c.workgroup != 512 && c.workgroup != 1024) || (c.schedule != 0 && c.schedule != 1) ||
-        c.order < 0 || c.order > 2 || (c.dynamic != 0 && c.dynamic != 1) || c.grid < 0)
+        c.order < 0 || c.order > 3 || (c.dynamic != 0 && c.dynamic != 1) || c.grid < 0)
         return BFLY_ERR_SHAPE;
     GroupPlan& P = *out;
     P = GroupPlan{};
@@ -236,6 +236,17 @@ int group_plan(const bfly_group_request* reqs, int nreq, const bfly_group_config
     }
     std::vector<int> ord(nreq);
     for (int r = 0; r < nreq; r++) ord[r] = r;
+    if (c.order == 3) {
+        // least total work first: the short requests complete early, which
+        // minimises the mean and median completion time of the group
+        auto total = [&](int r) {
+            const uint64_t m = group_local_len(reqs[r].kind, reqs[r].n);
+            uint32_t lg = 0;
+            while ((1u << lg) < m) lg++;
+            return (uint64_t)reqs[r].batch * m * (uint64_t)lg;
+        };
+        std::stable_sort(ord.begin(), ord.end(), [&](int a, int b) { return total(a) < total(b); });
+    }
     if (c.order == 1) {
         // most work per unit first: filled transforms * n' * log2 n'
         auto work = [&](int r) {

@@ -134,6 +134,12 @@ def grouped_pattern(reps: int = 20):
     ms_per_eager = group.time_ms(lambda: group.launch_per_request(reqs), reps=max(3, reps // 4))
     lat = group.latencies_us(g)
     lat_seq = group.per_request_latencies_us(reqs)
+    # the same configuration with the least-work-first unit order (latency)
+    cfg_lat = group.Config(tuned.best.workgroup, tuned.best.schedule, 3, tuned.best.dynamic, tuned.best.grid)
+    g_lat = group.Group(reqs, cfg_lat)
+    ms_lat = group.time_ms(g_lat.launch, reps=reps)
+    lat_sjf = group.latencies_us(g_lat)
+    g_lat.close()
     nbytes = g.algorithmic_bytes()
     transforms = g.transforms()
     kinds = [k for k, _, _ in specs]
@@ -152,6 +158,12 @@ def grouped_pattern(reps: int = 20):
         "pct_hbm_peak": round(100.0 * nbytes / (ms_group * 1e-3) / 1e9 / HBM_PEAK_GBS, 2),
         "latency_us_p50": round(float(np.percentile(lat, 50)), 2),
         "latency_us_p99": round(float(np.percentile(lat, 99)), 2),
+        "least_work_first": {
+            "config": cfg_lat.name(),
+            "ms": round(ms_lat, 4),
+            "latency_us_p50": round(float(np.percentile(lat_sjf, 50)), 2),
+            "latency_us_p99": round(float(np.percentile(lat_sjf, 99)), 2),
+        },
         "per_request_launches": {
             "ms_graph": round(ms_per_graph, 4),
             "ms_eager": round(ms_per_eager, 4),

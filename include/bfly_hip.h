@@ -210,7 +210,9 @@ typedef struct bfly_group_config {
     int32_t schedule;  /* radix schedule: 0 = radix 16 first, remainder radix last;
                           1 = remainder radix first */
     int32_t order;     /* unit order: 0 = request order; 1 = most work per unit
-                          first; 2 = round robin over requests */
+                          first; 2 = round robin over requests; 3 = requests
+                          with the least total work first (lowest median
+                          completion time) */
     int32_t dynamic;   /* 0 = workgroup w runs units w, w+grid, ...;
                           1 = workgroups take units from an atomic counter */
     int32_t grid;      /* workgroups launched; 0 = the resident capacity */

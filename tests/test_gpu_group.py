@@ -68,7 +68,7 @@ def mixed():
     return reqs, group.outputs(reqs)
 
 
-@pytest.mark.parametrize("wg,order,dynamic", list(itertools.product((512, 1024), (0, 1, 2), (0, 1))))
+@pytest.mark.parametrize("wg,order,dynamic", list(itertools.product((512, 1024), (0, 1, 2, 3), (0, 1))))
 def test_group_equals_per_request_launches_bitwise(mixed, wg, order, dynamic):
     import torch
     from bfly import group
@@ -166,6 +166,21 @@ def test_uneven_workload_512_requests_seed3():
     ms = group.time_ms(g.launch, reps=5)
     assert lat.max() < 20 * ms * 1e3
     g.close()
+
+
+def test_least_work_first_order_lowers_median_latency():
+    # order 3 puts the requests with the least total work first: on the
+    # uneven workload their completion stamps come early, so the median
+    # per-request latency drops well below that of most-work-first (order 1)
+    from bfly import group
+    reqs = group.make_requests(group.uneven_specs(), seed=3)
+    p50 = {}
+    for order in (1, 3):
+        g = group.Group(reqs, group.Config(order=order, dynamic=1))
+        g.launch()
+        p50[order] = float(np.percentile(group.latencies_us(g), 50))
+        g.close()
+    assert p50[3] < 0.75 * p50[1], p50
 
 
 def test_autotune_picks_a_gated_configuration():

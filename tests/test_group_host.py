@@ -77,7 +77,7 @@ def test_group_unit_counts():
 
 
 @pytest.mark.parametrize("bad_cfg", [
-    (256, 0, 0, 0, 0), (512, 2, 0, 0, 0), (512, 0, 3, 0, 0), (512, 0, -1, 0, 0),
+    (256, 0, 0, 0, 0), (512, 2, 0, 0, 0), (512, 0, 4, 0, 0), (512, 0, -1, 0, 0),
     (512, 0, 0, 2, 0), (512, 0, 0, 0, -5)])
 def test_group_config_guards(bad_cfg):
     s = bfly.stub()
