@@ -332,8 +332,14 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
     constexpr int STRIDE = lds_len(N1) + 1;
     constexpr int GROUPS = N2 / C;
     static_assert(N2 % C == 0, "column groups must tile N2");
+    // W_N^m = A[m mod 256] B[m / 256]: A = W_N^j and B = W_N^{256 j} are
+    // entries of the full table, gathered from LDS instead of scattered
+    // single-element reads of the 8N-byte table
+    constexpr int NA = 256, NB = (int)(((uint64_t)N1 * N2) / NA);
     __shared__ float2 lds[C * STRIDE];
+    __shared__ float2 tab[NA + NB];
     const int tid = threadIdx.x;
+    for (int i = tid; i < NA + NB; i += bfly_plan::kBlock) tab[i] = i < NA ? wN[i] : wN[(i - NA) * NA];
     const int c = tid % C, t = tid / C;
     const uint64_t b = blockIdx.x / GROUPS;
     const int n2 = (blockIdx.x % GROUPS) * C + c;
@@ -345,6 +351,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
 #pragma unroll
         for (int r = 0; r < R0; r++)
             v[m * R0 + r] = ld_stream(&src[(uint64_t)N2 * (t + m * T + r * (N1 / R0))]);
+    // the first LDS exchange of the FFT orders the table writes before any read
     fft_local<I1, T, DIR>(v, lds + c * STRIDE, tw1, t);
     float2* dst = ws + b * N + n2;
 #pragma unroll
@@ -352,7 +359,10 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
 #pragma unroll
         for (int r = 0; r < RL; r++) {
             const int k1 = t + m * T + r * (N1 / RL);
-            st_stream(&dst[(uint64_t)N2 * k1], cmul_tw<DIR>(v[m * RL + r], wN[n2 * k1]));
+            const int e = (n2 * k1) & (int)(N - 1);
+            const float2 a = tab[e & (NA - 1)], w = tab[NA + (e >> 8)];
+            const float2 wv = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+            st_stream(&dst[(uint64_t)N2 * k1], cmul_tw<DIR>(v[m * RL + r], wv));
         }
 }
 
