@@ -12,6 +12,9 @@ the whole batch ([4096][8192] float32, interleaved), inputs resident in HBM.  Pr
   chosen         every kernel variant of this shape timed on the same buffers
                  and gated against FP64; the fastest passing one, its speedup
                  over the plain one-shot variant and its relative error
+  rows           the other hot-path rows at their shapes: C2C n=1024 b=64
+                 latency, R2C / C2R 4096x4096 (reference CPU rows beside
+                 them), long C2C 65536 (b=64, 256) and 16384 (b=1024)
   patterns       the uneven batched workload (512 requests) as one grouped
                  persistent launch under hipGraph vs per-request launches:
                  transforms/s, GB/s, p50/p99 per-request latency; the
@@ -103,6 +106,84 @@ def cpu_baseline(budget_s: float):
                   f"the N=4096 C2C workload through the reference's refFftF32 (FP64, one "
                   f"thread) in {dt:.1f} s",
     }
+
+
+def secondary_rows(cpu_seconds: float, reps: int = 20):
+    """The other hot-path rows at their BASELINE shapes, inputs resident and
+    rotated over 4 buffer sets (>= 256 MB MALL): cfg1 C2C latency, R2C / C2R
+    4096x4096 (with the reference's refR2CF32 / refC2RF32 timed on one host
+    core over a bounded sample), long C2C at 65536 (b = 64 and 256) and
+    16384 (b = 1024).  Kernel time by HIP events on the launch stream."""
+    import numpy as np
+    import torch
+    from bfly import _lib, ops
+    import oracle
+
+    lib = _lib.hip()
+    dev = torch.device("cuda:0")
+    sid = lambda: torch.cuda.current_stream().cuda_stream  # noqa: E731
+
+    def timed(fn):
+        for i in range(5):
+            fn(i)
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for i in range(reps):
+            fn(i)
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps
+
+    def row(ms, nbytes, transforms):
+        gbs = nbytes / (ms * 1e-3) / 1e9
+        return {"us": round(ms * 1e3, 2), "GB_per_s": round(gbs, 1), "frac_of_8TBs": round(gbs / HBM_PEAK_GBS, 4),
+                "transforms_per_s": round(transforms / (ms * 1e-3), 1), "algorithmic_bytes": int(nbytes)}
+
+    out = {}
+    # cfg1: C2C n=1024 b=64 (latency-bound)
+    n, b = 1024, 64
+    xs = [torch.rand(b, 2 * n, device=dev) for _ in range(4)]
+    ys = [torch.empty_like(xs[0]) for _ in range(4)]
+    out["c2c_n1024_b64"] = row(timed(lambda i: ops.c2c_raw(xs[i % 4], ys[i % 4], n, b, -1)), 16 * n * b, b)
+    # R2C / C2R 4096 x 4096
+    n, b = 4096, 4096
+    xr = [torch.rand(b, n, device=dev) * 2 - 1 for _ in range(4)]
+    hs = [torch.empty(b, n + 2, device=dev) for _ in range(4)]
+    for kind, name, src, dst in ((1, "r2c_n4096_b4096", xr, hs), (2, "c2r_n4096_b4096", hs, xr)):
+        tab = ops.table(kind, n, dev)
+        fn = (lambda k, t, a, c: lambda i: _lib.check(lib.bfly_real_ex(
+            k, a[i % 4].data_ptr(), c[i % 4].data_ptr(), n, b, n // 2 + 1, n + 2, _lib.VARIANT_AUTO,
+            t.data_ptr(), sid()), "real"))(kind, tab, src, dst)
+        out[name] = row(timed(fn), (4 * n + 4 * (n + 2)) * b, b)
+    del xr, hs
+    r = oracle.ref()
+    if r is not None and cpu_seconds > 0:
+        x = oracle.uniform_input(1, (64, n))
+        h = oracle.r2c_f64(x, n).astype(np.float32)
+        h[:, 1] = 0
+        h[:, n + 1] = 0
+        for name, fn, a, width in (("r2c_n4096_b4096", r.ref_r2c_f32, x, n + 2),
+                                   ("c2r_n4096_b4096", r.ref_c2r_f32, h, n)):
+            o = np.empty(width, np.float32)
+            t0, done = time.perf_counter(), 0
+            while time.perf_counter() - t0 < cpu_seconds:
+                fn(np.ascontiguousarray(a[done % 64]), o, n)
+                done += 1
+            dt = time.perf_counter() - t0
+            out[name]["cpu_baseline"] = {
+                "value": round((4 * n + 4 * (n + 2)) * done / dt / 1e9, 4), "unit": "GB/s", "cores": 1,
+                "kind": "reference", "sample": f"{done} rows of n={n} through the reference's "
+                f"{'refR2CF32' if name.startswith('r2c') else 'refC2RF32'} in {dt:.1f} s"}
+    # long C2C
+    for n, b in ((65536, 64), (65536, 256), (16384, 1024)):
+        xs = [torch.rand(b, 2 * n, device=dev) for _ in range(4)]
+        ys = [torch.empty_like(xs[0]) for _ in range(4)]
+        work = ops.workspace(0, n, b, dev)
+        out[f"c2c_n{n}_b{b}"] = row(timed(lambda i: ops.c2c_raw(xs[i % 4], ys[i % 4], n, b, -1, work)),
+                                    16 * n * b, b)
+        del xs, ys, work
+    return out
 
 
 def grouped_pattern(reps: int = 20):
@@ -391,6 +472,7 @@ def main():
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
     }
     if not args.no_patterns:
+        line["rows"] = secondary_rows(0 if args.no_cpu else 1.5)
         line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern()}
     print(json.dumps(line))
 
