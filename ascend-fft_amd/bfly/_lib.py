@@ -34,7 +34,8 @@ VARIANT_STREAM_LT = 5
 VARIANT_STREAM2 = 6
 VARIANT_FOUR_STEP = 7
 VARIANT_SPLIT_MFMA = 8
-VARIANT_LAST = 8
+VARIANT_SPLIT_MFMA_STREAM = 9
+VARIANT_LAST = 9
 
 KIND_C2C = 0
 KIND_R2C = 1

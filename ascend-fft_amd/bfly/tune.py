@@ -22,6 +22,7 @@ VARIANT_NAMES = {
     _lib.VARIANT_STREAM_LT: "stream_lds_twiddles",
     _lib.VARIANT_STREAM2: "stream_depth2",
     _lib.VARIANT_SPLIT_MFMA: "split_fp16_mfma",
+    _lib.VARIANT_SPLIT_MFMA_STREAM: "split_fp16_mfma_stream",
 }
 
 
@@ -29,7 +30,8 @@ def variants(kind: int, n: int) -> list[int]:
     """C2C kernel variants of length n; the first is the plain one-shot grid."""
     if kind != _lib.KIND_C2C or n > 4096:
         return [_lib.VARIANT_PLAIN]
-    return [v for v in VARIANT_NAMES if v != _lib.VARIANT_SPLIT_MFMA or n in (1024, 2048, 4096)]
+    mfma = (_lib.VARIANT_SPLIT_MFMA, _lib.VARIANT_SPLIT_MFMA_STREAM)
+    return [v for v in VARIANT_NAMES if v not in mfma or n in (1024, 2048, 4096)]
 
 
 def rel_l2_fp64(got: np.ndarray, x: np.ndarray, n: int, sign: int = -1) -> float:

@@ -564,7 +564,8 @@ namespace bfly_kern {
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s) {
-    if (variant == BFLY_VARIANT_SPLIT_MFMA) return c2c_mfma(in, out, n, batch, sign, table, s);
+    if (variant == BFLY_VARIANT_SPLIT_MFMA || variant == BFLY_VARIANT_SPLIT_MFMA_STREAM)
+        return c2c_mfma(in, out, n, batch, sign, variant == BFLY_VARIANT_SPLIT_MFMA_STREAM, table, s);
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     const float2* tw = reinterpret_cast<const float2*>(table);

@@ -203,51 +203,44 @@ __device__ __forceinline__ void read16_lds(float2 (&v)[16], const float2* lds, i
         }
 }
 
-// Plans [16, 16, R2]: two MFMA passes, then radix R2 -- on the matrix cores
-// when R2 = 16, on the vector ALUs otherwise.
+// The lane's pass-0 inputs x[j + r N/16] straight from global memory.
+template <int N>
+__device__ __forceinline__ void load16_global(float2 (&v)[16], const float2* __restrict__ src, bool active,
+                                              int w, int lane) {
+    const int h = lane >> 4, c = lane & 15;
+#pragma unroll
+    for (int i = 0; i < 4; i++)
+#pragma unroll
+        for (int rr = 0; rr < 4; rr++)
+            v[4 * i + rr] = active ? ld_io<true>(&src[64 * w + 16 * i + c + (4 * h + rr) * (N / 16)])
+                                   : make_float2(0.f, 0.f);
+}
+
+// One transform of plan [16, 16, R2] whose pass-0 inputs are in v: two MFMA
+// passes, then radix R2 -- on the matrix cores when R2 = 16, on the vector
+// ALUs otherwise -- and the natural-order store to dst (when active).
+// `lds` is the transform's region, `s_exp` its per-wave exponent slots.
 template <int I, int DIR>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __restrict__ in,
-                                                              float2* __restrict__ out,
-                                                              const float2* __restrict__ tw,
-                                                              uint32_t batch) {
+__device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int* s_exp,
+                                               const float2* __restrict__ tw, const half8 (&ahi)[2],
+                                               const half8 (&alo)[2], int t, float2* __restrict__ dst,
+                                               bool active) {
     using PL = PlanOf<I>;
-    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int N = PL::N, T = PL::T;
     constexpr int R2 = bfly_plan::kPlans[I].rad[2];
     static_assert(bfly_plan::kPlans[I].nrad == 3 && bfly_plan::kPlans[I].rad[0] == 16 &&
                       bfly_plan::kPlans[I].rad[1] == 16 && T % 64 == 0,
                   "plans [16, 16, R2] with whole waves per transform");
     constexpr int WAVES = T / 64;  // waves per transform
-    __shared__ float2 lds_all[B * lds_len(N)];
-    __shared__ int s_exp[B * WAVES];
-    const int tid = threadIdx.x;
-    const int bl = tid / T, t = tid % T;
     const int w = t >> 6, lane = t & 63, h = lane >> 4, c = lane & 15;
-    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
-    const bool active = tr < batch;
-    float2* lds = lds_all + bl * lds_len(N);
-    half8 ahi[2], alo[2];
-    dft16_operands<DIR>(lane, ahi, alo);
-
-    float2 v[16];
-    if (active) {
-        const float2* src = in + tr * N;
-#pragma unroll
-        for (int i = 0; i < 4; i++)
-#pragma unroll
-            for (int rr = 0; rr < 4; rr++)
-                v[4 * i + rr] = ld_io<true>(&src[64 * w + 16 * i + c + (4 * h + rr) * (N / 16)]);
-    } else {
-#pragma unroll
-        for (int i = 0; i < 16; i++) v[i] = make_float2(0.f, 0.f);
-    }
     // one scale per transform: the largest input component over its waves
     constexpr int PM = R2 == 16 ? 3 : 2;  // matrix passes
     int et = wave_max_exponent(v);
     if constexpr (WAVES > 1) {
-        if (lane == 0) s_exp[bl * WAVES + w] = et;
+        if (lane == 0) s_exp[w] = et;
         __syncthreads();
 #pragma unroll
-        for (int k = 0; k < WAVES; k++) et = max(et, s_exp[bl * WAVES + k]);
+        for (int k = 0; k < WAVES; k++) et = max(et, s_exp[k]);
     }
     int sh = 11 - et - 5 * (PM - 1);
     sh = sh > 126 ? 126 : (sh < -126 ? -126 : sh);
@@ -268,7 +261,6 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
         __syncthreads();
         pass16_mfma<N, 256, DIR, true>(v, lds, tw2, w, lane, ahi, alo);
         if (!active) return;
-        float2* dst = out + tr * N;
 #pragma unroll
         for (int i = 0; i < 4; i++)
 #pragma unroll
@@ -284,6 +276,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
         for (int m = 0; m < M; m++)
 #pragma unroll
             for (int r = 0; r < R2; r++) v[m * R2 + r] = lds[lds_idx(t + m * T + r * (N / R2))];
+        __syncthreads();  // a caller may reuse `lds` right after
         constexpr int Ns = 256;
 #pragma unroll
         for (int m = 0; m < M; m++) {
@@ -293,7 +286,6 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
             dft<R2, DIR>(&v[m * R2]);
         }
         if (!active) return;
-        float2* dst = out + tr * N;
 #pragma unroll
         for (int m = 0; m < M; m++)
 #pragma unroll
@@ -302,11 +294,95 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
     }
 }
 
+// One-shot grid: B = 256/T transforms per workgroup.
+template <int I, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __restrict__ in,
+                                                              float2* __restrict__ out,
+                                                              const float2* __restrict__ tw,
+                                                              uint32_t batch) {
+    constexpr int N = PlanOf<I>::N, T = PlanOf<I>::T, B = bfly_plan::kBlock / T, WAVES = T / 64;
+    __shared__ float2 lds_all[B * lds_len(N)];
+    __shared__ int s_exp[B * WAVES];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
+    const bool active = tr < batch;
+    half8 ahi[2], alo[2];
+    dft16_operands<DIR>(t & 63, ahi, alo);
+    float2 v[16];
+    load16_global<N>(v, in + tr * N, active, t >> 6, t & 63);
+    mfma_transform<I, DIR>(v, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, ahi, alo, t, out + tr * N,
+                           active);
+}
+
+// Streaming form: a resident grid walks the batch and loads each
+// workgroup's next transforms while the current ones compute.
+template <int I, int DIR>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma_stream(const float2* __restrict__ in,
+                                                                     float2* __restrict__ out,
+                                                                     const float2* __restrict__ tw,
+                                                                     uint32_t batch) {
+    constexpr int N = PlanOf<I>::N, T = PlanOf<I>::T, B = bfly_plan::kBlock / T, WAVES = T / 64;
+    __shared__ float2 lds_all[B * lds_len(N)];
+    __shared__ int s_exp[B * WAVES];
+    const int tid = threadIdx.x;
+    const int bl = tid / T, t0 = tid % T;
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    half8 ahi[2], alo[2];
+    dft16_operands<DIR>(t0 & 63, ahi, alo);
+    float2 cur[16], nxt[16];
+    uint32_t g = blockIdx.x;
+    if (g < groups) {
+        const uint64_t tr = (uint64_t)g * B + bl;
+        load16_global<N>(cur, in + tr * N, tr < batch, t0 >> 6, t0 & 63);
+    }
+    for (; g < groups; g += gridDim.x) {
+        int t = t0;  // opaque per iteration: keeps index-derived addresses in the loop
+        asm volatile("" : "+v"(t));
+        const uint32_t gn = g + gridDim.x;
+        if (gn < groups) {
+            const uint64_t trn = (uint64_t)gn * B + bl;
+            load16_global<N>(nxt, in + trn * N, trn < batch, t >> 6, t & 63);
+        }
+        const uint64_t tr = (uint64_t)g * B + bl;
+        mfma_transform<I, DIR>(cur, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, ahi, alo, t,
+                               out + tr * N, tr < batch);
+#pragma unroll
+        for (int i = 0; i < 16; i++) cur[i] = nxt[i];
+    }
+}
+
+int resident_blocks(const void* f) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, bfly_plan::kBlock, 0) != hipSuccess)
+        return 0;
+    return cus * (per > 0 ? per : 1);
+}
+
 template <int I>
 hipError_t launch_mfma(const float2* in, float2* out, const float2* tw, uint32_t batch, int sign,
-                       hipStream_t s) {
+                       bool stream, hipStream_t s) {
     constexpr int B = bfly_plan::kBlock / PlanOf<I>::T;
-    const dim3 grid((unsigned)((batch + B - 1) / B));
+    const uint64_t groups = (batch + B - 1) / B;
+    if (stream) {
+        const void* f = sign < 0 ? reinterpret_cast<const void*>(&k_c2c_mfma_stream<I, -1>)
+                                 : reinterpret_cast<const void*>(&k_c2c_mfma_stream<I, 1>);
+        static int cap[2] = {0, 0};
+        int& c = cap[sign < 0 ? 0 : 1];
+        if (c == 0) c = resident_blocks(f);
+        if (c <= 0) return hipErrorInvalidValue;
+        const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
+        if (sign < 0)
+            hipLaunchKernelGGL((k_c2c_mfma_stream<I, -1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw,
+                               batch);
+        else
+            hipLaunchKernelGGL((k_c2c_mfma_stream<I, 1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw,
+                               batch);
+        return hipGetLastError();
+    }
+    const dim3 grid((unsigned)groups);
     if (sign < 0)
         hipLaunchKernelGGL((k_c2c_mfma<I, -1>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
     else
@@ -320,15 +396,15 @@ constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
 
 namespace bfly_kern {
 
-hipError_t c2c_mfma(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+hipError_t c2c_mfma(const float* in, float* out, uint32_t n, uint32_t batch, int sign, bool stream,
                     const float* table, hipStream_t s) {
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     const float2* tw = reinterpret_cast<const float2*>(table);
     switch (n) {
-        case 1024: return launch_mfma<idx(1024)>(i2, o2, tw, batch, sign, s);
-        case 2048: return launch_mfma<idx(2048)>(i2, o2, tw, batch, sign, s);
-        case 4096: return launch_mfma<idx(4096)>(i2, o2, tw, batch, sign, s);
+        case 1024: return launch_mfma<idx(1024)>(i2, o2, tw, batch, sign, stream, s);
+        case 2048: return launch_mfma<idx(2048)>(i2, o2, tw, batch, sign, stream, s);
+        case 4096: return launch_mfma<idx(4096)>(i2, o2, tw, batch, sign, stream, s);
         default: return hipErrorInvalidValue;
     }
 }

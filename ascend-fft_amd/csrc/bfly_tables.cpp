@@ -149,7 +149,8 @@ int check_execute(int kind, int sign, const void* in, const void* out, uint32_t 
     if (!grid_ok(kind, n, batch)) return BFLY_ERR_SHAPE;
     if (!table) return BFLY_ERR_TABLE;
     if (needs_work(kind, n, batch, variant) && !work) return BFLY_ERR_SHAPE;
-    if (kind == BFLY_KIND_C2C && variant == BFLY_VARIANT_SPLIT_MFMA && !split_mfma_ok(n))
+    if (kind == BFLY_KIND_C2C &&
+        (variant == BFLY_VARIANT_SPLIT_MFMA || variant == BFLY_VARIANT_SPLIT_MFMA_STREAM) && !split_mfma_ok(n))
         return BFLY_ERR_SHAPE;
     return BFLY_OK;
 }

@@ -129,7 +129,9 @@ BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, i
  *                        the matrix cores in split FP16 (hi + lo operands,
  *                        FP32 accumulation, per-butterfly power-of-two
  *                        scaling); FP32 twiddles.  Relative L2 error vs FP64
- *                        about 2x that of the FP32 variants, within 1e-5
+ *                        about 3x that of the FP32 variants, within 1e-5
+ *   BFLY_VARIANT_SPLIT_MFMA_STREAM  the same on a resident grid that
+ *                        prefetches each workgroup's next transforms
  * For n <= 4096 FOUR_STEP means PLAIN; above 4096 every other id selects the
  * length's default path (one workgroup per transform up to 16384, four-step
  * above). */
@@ -142,7 +144,8 @@ BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, i
 #define BFLY_VARIANT_STREAM2 6
 #define BFLY_VARIANT_FOUR_STEP 7
 #define BFLY_VARIANT_SPLIT_MFMA 8
-#define BFLY_VARIANT_LAST 8
+#define BFLY_VARIANT_SPLIT_MFMA_STREAM 9
+#define BFLY_VARIANT_LAST 9
 BFLY_API int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
                          int variant, const float* table, void* work, void* stream);
 

@@ -169,7 +169,7 @@ def test_python_modules_import_and_variant_ids_match_header():
         importlib.import_module(m)
     text = HEADER.read_text()
     for name in ("AUTO", "PLAIN", "STREAM", "PLAIN_SWZ", "STREAM_SWZ", "STREAM_LT", "STREAM2",
-                 "FOUR_STEP", "SPLIT_MFMA", "LAST"):
+                 "FOUR_STEP", "SPLIT_MFMA", "SPLIT_MFMA_STREAM", "LAST"):
         m = re.search(rf"#define BFLY_VARIANT_{name} (\d+)", text)
         assert m and int(m.group(1)) == getattr(_lib, f"VARIANT_{name}"), name
     for name in ("OK", "ERR_NOT_LOADED", "ERR_TABLE", "ERR_ALLOC", "ERR_H2D", "ERR_LAUNCH",

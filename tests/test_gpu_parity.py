@@ -98,11 +98,14 @@ def test_split_mfma_variant_error_vs_fp64(n):
     # the FP32 paths; its error is stated against FP64 and stays within a
     # few times the FP32 Stockham's (about 1.3e-7 at these lengths).
     from bfly import _lib
-    for b, sign in ((1, -1), (7, 1), (64, -1), (1000, 1)):
+    for b, sign in ((1, -1), (7, 1), (64, -1), (1000, 1), (3001, -1)):
         x = oracle.uniform_input(21, (b, 2 * n))
         got = _run_variant(x, n, b, sign, _lib.VARIANT_SPLIT_MFMA)
         e = gate_c2c(got, x, n, sign)
         assert e < 2e-6, f"n={n} b={b} rel-L2 {e:.3e}"
+        # the resident-grid form runs the same per-transform code
+        got_s = _run_variant(x, n, b, sign, _lib.VARIANT_SPLIT_MFMA_STREAM)
+        assert np.array_equal(got_s, got), (n, b, sign)
 
 
 @pytest.mark.parametrize("n", [1024, 4096])
