@@ -453,7 +453,7 @@ hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw, const 
     static int cap = 0;
     if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>));
     if (cap <= 0) return hipErrorInvalidValue;
-    const dim3 grid((unsigned)(groups < (uint64_t)cap ? groups : (uint64_t)cap));
+    const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)cap));
     hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, tw2g,
                        batch);
     return hipGetLastError();
