@@ -95,7 +95,8 @@ int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t bat
     if (int g = bfly_host::check_execute(kind, sign, in, out, n, batch, table, nullptr)) return g;
     if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
     if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_real_variant(n, batch);
-    if (variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM) return BFLY_ERR_SHAPE;
+    if (variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM && variant != BFLY_VARIANT_STREAM_LT)
+        return BFLY_ERR_SHAPE;
     return launched(bfly_kern::real(kind, in, out, n, batch, bins, stride, variant, table, S(stream)));
 }
 

@@ -264,7 +264,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream(const float* _
         int t2 = tid;
         asm volatile("" : "+v"(t2));
         if (gn < groups) r2c_load<I, bfly_plan::kBlock>(nxt, in, batch, gn, t2);
-        r2c_finish<I, bfly_plan::kBlock>(cur, out, tw, wn, batch, kout, ostride, g, lds, t2);
+        r2c_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, kout, ostride, g, lds, t2);
         __syncthreads();  // the post-twiddle reads of `lds` end before the next FFT writes it
 #pragma unroll
         for (int i = 0; i < P; i++) cur[i] = nxt[i];
@@ -308,8 +308,69 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream(const float* _
         int t2 = tid;  // opaque per iteration (see k_r2c_stream)
         asm volatile("" : "+v"(t2));
         if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
-        c2r_finish<I, bfly_plan::kBlock>(cur, out, tw, wn, batch, scale, g, lds, t2);
+        c2r_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, g, lds, t2);
         __syncthreads();  // the last exchange's reads end before the next staging writes
+#pragma unroll
+        for (int i = 0; i <= P; i++) cur[i] = nxt[i];
+    }
+}
+
+// R2C / C2R, streaming form with LDS twiddles (the FFT's two-level tables
+// and the split twiddle W_n^k as a product of two small tables), launched
+// about 1.5 row groups per workgroup (see bfly_plan::stream_grid).
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream_lt(const float* __restrict__ in,
+                                                                     float* __restrict__ out,
+                                                                     const float2* __restrict__ tw,
+                                                                     const float2* __restrict__ wn,
+                                                                     uint32_t batch, uint32_t kout,
+                                                                     uint32_t ostride) {
+    constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
+    __shared__ float2 lds[B * lds_len(M)];
+    __shared__ float2 ltw[real_lds_tw_len<I>()];
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    const int tid = threadIdx.x;
+    const RealTwLds<I> tws = real_lds_tw<I, bfly_plan::kBlock>(ltw, tw, wn, tid);
+    // the FFT's first exchange barrier orders the table writes before use
+    float2 cur[P], nxt[P];
+    uint32_t g = blockIdx.x;
+    if (g < groups) r2c_load<I, bfly_plan::kBlock>(cur, in, batch, g, tid);
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t gn = g + gridDim.x;
+        int t2 = tid;  // opaque per iteration (see k_r2c_stream)
+        asm volatile("" : "+v"(t2));
+        if (gn < groups) r2c_load<I, bfly_plan::kBlock>(nxt, in, batch, gn, t2);
+        r2c_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, kout, ostride, g, lds, t2);
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < P; i++) cur[i] = nxt[i];
+    }
+}
+
+template <int I>
+__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float* __restrict__ in,
+                                                                     float* __restrict__ out,
+                                                                     const float2* __restrict__ tw,
+                                                                     const float2* __restrict__ wn,
+                                                                     uint32_t batch, float scale,
+                                                                     uint32_t kin, uint32_t istride) {
+    constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
+    __shared__ float2 lds[B * lds_len(M)];
+    __shared__ float2 ltw[real_lds_tw_len<I>()];
+    const uint32_t groups = (uint32_t)((batch + B - 1) / B);
+    const int tid = threadIdx.x;
+    const RealTwLds<I> tws = real_lds_tw<I, bfly_plan::kBlock>(ltw, tw, wn, tid);
+    __syncthreads();  // the pre-twiddle reads the tables before the FFT's first barrier
+    float2 cur[P + 1], nxt[P + 1];
+    uint32_t g = blockIdx.x;
+    if (g < groups) c2r_load<I, bfly_plan::kBlock>(cur, in, batch, kin, istride, g, tid);
+    for (; g < groups; g += gridDim.x) {
+        const uint32_t gn = g + gridDim.x;
+        int t2 = tid;  // opaque per iteration (see k_r2c_stream)
+        asm volatile("" : "+v"(t2));
+        if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
+        c2r_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, scale, g, lds, t2);
+        __syncthreads();
 #pragma unroll
         for (int i = 0; i <= P; i++) cur[i] = nxt[i];
     }
@@ -497,6 +558,25 @@ hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, 
                        uint32_t batch, uint32_t bins, uint32_t stride, int variant, hipStream_t s) {
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
     const uint64_t groups = (batch + B - 1) / B;
+    if constexpr (PlanOf<I>::N >= 128) {
+        if (variant == BFLY_VARIANT_STREAM_LT) {
+            const void* f = kind == 1 ? reinterpret_cast<const void*>(&k_r2c_stream_lt<I>)
+                                      : reinterpret_cast<const void*>(&k_c2r_stream_lt<I>);
+            static int cap[2] = {0, 0};
+            int& c = cap[kind == 1 ? 0 : 1];
+            if (c == 0) c = resident_blocks(f);
+            if (c <= 0) return hipErrorInvalidValue;
+            const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)c));
+            if (kind == 1)
+                hipLaunchKernelGGL((k_r2c_stream_lt<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
+                                   batch, bins, stride);
+            else
+                hipLaunchKernelGGL((k_c2r_stream_lt<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
+                                   batch, 1.0f / (float)(2 * PlanOf<I>::N), bins, stride);
+            return hipGetLastError();
+        }
+    }
+    if (variant == BFLY_VARIANT_STREAM_LT) variant = BFLY_VARIANT_STREAM;
     if (variant == BFLY_VARIANT_STREAM) {
         const void* f = kind == 1 ? reinterpret_cast<const void*>(&k_r2c_stream<I>)
                                   : reinterpret_cast<const void*>(&k_c2r_stream<I>);

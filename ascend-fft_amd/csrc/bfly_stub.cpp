@@ -116,7 +116,8 @@ int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t bat
                                          nullptr))
         return e;
     if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
-    if (variant != BFLY_VARIANT_AUTO && variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM)
+    if (variant != BFLY_VARIANT_AUTO && variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM &&
+        variant != BFLY_VARIANT_STREAM_LT)
         return BFLY_ERR_SHAPE;
     return launch();
 }

@@ -171,12 +171,14 @@ int auto_variant(uint32_t n, uint32_t batch) {
     return BFLY_VARIANT_PLAIN;
 }
 
-// Real transforms: the streaming form once the batch fills several rounds of
-// resident workgroups (measured: R2C n=4096 b=4096 33.0 vs 37.8 us, C2R
-// n=256 b=65536 30.1 vs 34.5 us); at n = 8192 (one transform per workgroup)
-// the one-shot grid is as fast or faster.
+// Real transforms, once the batch fills several rounds of resident
+// workgroups: streaming with LDS twiddles from n = 1024 (measured, MI355X:
+// R2C n=8192 b=4096 54.3 us vs 74.1 one-shot / 74.8 streaming; C2R n=4096
+// b=4096 31.2 vs 35.1 / 33.9), the register-table streaming form below
+// (C2R n=256 b=65536 30.6 vs 32.3 LDS twiddles / 34.4 one-shot).
 int auto_real_variant(uint32_t n, uint32_t batch) {
-    return (n <= 4096 && (uint64_t)n * batch >= (1ull << 22)) ? BFLY_VARIANT_STREAM : BFLY_VARIANT_PLAIN;
+    if ((uint64_t)n * batch < (1ull << 22)) return BFLY_VARIANT_PLAIN;
+    return n >= 1024 ? BFLY_VARIANT_STREAM_LT : BFLY_VARIANT_STREAM;
 }
 
 bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride) {

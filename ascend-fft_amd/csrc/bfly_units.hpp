@@ -106,6 +106,78 @@ __device__ __forceinline__ void c2c_unit(const float2* __restrict__ in, float2* 
     }
 }
 
+// --------------------------------------------------- real twiddle sources
+// Where a real unit takes its twiddles from: the plan's pass tables (for
+// the n/2-point FFT) and W_n^k, k <= n/2 (for the split twiddle).
+// RealTwGlobal reads both tables from global memory; RealTwLds reads LDS
+// copies made once per workgroup: the two-level tables of the FFT
+// (stockham_lt) and W_n^k = wlo[k mod 64] * whi[k / 64].
+template <int I, int SC>
+struct RealTwGlobal {
+    const float2* __restrict__ tw;
+    const float2* __restrict__ wn;
+    template <int T, int DIR>
+    __device__ __forceinline__ void fft(float2 (&v)[bfly_plan::kPoints], float2* L, int t) const {
+        fft_local<I, T, DIR, SC>(v, L, tw, t);
+    }
+    __device__ __forceinline__ float2 w(int k) const { return wn[k]; }
+};
+
+template <int I>
+struct RealTwLds {
+    const float2* tlo;  // W_M^j, j < 64
+    const float2* thi;  // W_M^{64 j}, j < M / 64
+    const float2* t1;   // second pass's table, [r 16 + k]
+    const float2* wlo;  // W_n^j, j < 64
+    const float2* whi;  // W_n^{64 j}, 64 j <= n / 2
+    template <int T, int DIR>
+    __device__ __forceinline__ void fft(float2 (&v)[bfly_plan::kPoints], float2* L, int t) const {
+        run_plan_lt<I, T, DIR, LX_PAD>(v, L, 0, tlo, thi, t1, t, std::make_index_sequence<kPlans[I].nrad>{});
+    }
+    __device__ __forceinline__ float2 w(int k) const {
+        const float2 a = wlo[k & 63], b = whi[k >> 6];
+        return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+};
+
+// LDS entries RealTwLds needs for plan I (n = 2 N_I), and the copy that fills
+// them from the global tables of the real kind: tw (pass tables) and wn
+// (W_n^k, k <= n/2; W_n^e for e > n/2 is conj W_n^{n-e}).
+template <int I>
+constexpr int real_lds_tw_len() {
+    constexpr int M = kPlans[I].n;
+    return 64 + M / 64 + bfly_plan::kPoints * kPlans[I].rad[1] + 64 + M / 64 + 1;
+}
+template <int I, int WG>
+__device__ __forceinline__ RealTwLds<I> real_lds_tw(float2* buf, const float2* __restrict__ tw,
+                                                    const float2* __restrict__ wn, int tid) {
+    constexpr int M = kPlans[I].n, n = 2 * M, R1 = kPlans[I].rad[1];
+    static_assert(M >= 128 && kPlans[I].rad[0] == 16, "two-level tables need M >= 128");
+    float2* tlo = buf;
+    float2* thi = tlo + 64;
+    float2* t1 = thi + M / 64;
+    float2* wlo = t1 + 16 * R1;
+    float2* whi = wlo + 64;
+    auto wfull = [&](int e) {  // W_n^e, 0 <= e < n
+        if (e <= M) return wn[e];
+        const float2 c = wn[n - e];
+        return make_float2(c.x, -c.y);
+    };
+    for (int i = tid; i < real_lds_tw_len<I>(); i += WG) {
+        if (i < 64) tlo[i] = wfull(2 * i);
+        else if (i < 64 + M / 64) thi[i - 64] = wfull(128 * (i - 64));
+        else if (i < 64 + M / 64 + 16 * R1) {
+            const int j = i - 64 - M / 64;  // stage-table entry [k R1 + r]
+            t1[(j % R1) * 16 + j / R1] = tw[j];
+        } else if (i < 64 + M / 64 + 16 * R1 + 64) {
+            wlo[i - 64 - M / 64 - 16 * R1] = wn[i - 64 - M / 64 - 16 * R1];
+        } else {
+            whi[i - 128 - M / 64 - 16 * R1] = wn[64 * (i - 128 - M / 64 - 16 * R1)];
+        }
+    }
+    return RealTwLds<I>{tlo, thi, t1, wlo, whi};
+}
+
 // ------------------------------------------------------------------ R2C
 // z[j] = x[2j] + i x[2j+1], Z = FFT_M(z), M = n/2; for k = 0..M:
 //   E = (Z[k] + conj Z[M-k]) / 2,  O = (Z[k] - conj Z[M-k]) / (2i),
@@ -137,18 +209,17 @@ __device__ __forceinline__ void r2c_load(float2 (&v)[bfly_plan::kPoints], const 
 // Compute phase: FFT_M, spectrum parked in LDS, split post-twiddle, store.
 // Ends with LDS reads still in flight in other threads: a caller that reuses
 // `lds` synchronises first.
-template <int I, int WG, int SC = 0>
+template <int I, int WG, int SC = 0, class TWS = RealTwGlobal<I, SC>>
 __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], float* __restrict__ out,
-                                           const float2* __restrict__ tw, const float2* __restrict__ wn,
-                                           uint32_t batch, uint32_t kout, uint32_t ostride, uint32_t g,
-                                           float2* lds, int tid) {
+                                           const TWS& tws, uint32_t batch, uint32_t kout, uint32_t ostride,
+                                           uint32_t g, float2* lds, int tid) {
     using PL = PlanOf<I, SC>;
     constexpr int M = PL::N, T = PL::T, B = WG / T, RL = PL::RL, P = PL::P;
     static_assert(WG % T == 0, "a work unit holds whole transforms");
     const int bl = tid / T, t = tid % T;
     const uint64_t tr = (uint64_t)g * B + bl;
     float2* L = lds + bl * lds_len(M);
-    fft_local<I, T, -1, SC>(v, L, tw, t);
+    tws.template fft<T, -1>(v, L, t);
     // Every LDS read of the last exchange completed before the barrier that
     // ends it, so the spectrum can be parked immediately.
 #pragma unroll
@@ -167,7 +238,7 @@ __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], floa
                 const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
                 // O = (a - conj b) / (2i) = (0.5 (a.y + b.y), -0.5 (a.x - b.x))
                 const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
-                const float2 w = wn[k];
+                const float2 w = tws.w(k);
                 const float2 wo = cmul_tw<-1>(o, w);
                 st_io<(T >= 16)>(&dst[k], cadd(e, wo));
             }
@@ -182,7 +253,7 @@ __device__ __forceinline__ void r2c_unit(const float* __restrict__ in, float* __
                                          float2* lds, int tid) {
     float2 v[bfly_plan::kPoints];
     r2c_load<I, WG, SC>(v, in, batch, g, tid);
-    r2c_finish<I, WG, SC>(v, out, tw, wn, batch, kout, ostride, g, lds, tid);
+    r2c_finish<I, WG, SC>(v, out, RealTwGlobal<I, SC>{tw, wn}, batch, kout, ostride, g, lds, tid);
 }
 
 // ------------------------------------------------------------------ C2R
@@ -212,10 +283,10 @@ __device__ __forceinline__ void c2r_load(float2 (&q)[bfly_plan::kPoints + 1], co
 // Compute phase: bins staged in LDS, Hermitian pre-twiddle, inverse FFT_M,
 // reals scaled and stored.  Starts by writing `lds`: a caller that reuses it
 // synchronises first.
-template <int I, int WG, int SC = 0>
+template <int I, int WG, int SC = 0, class TWS = RealTwGlobal<I, SC>>
 __device__ __forceinline__ void c2r_finish(float2 (&q)[bfly_plan::kPoints + 1], float* __restrict__ out,
-                                           const float2* __restrict__ tw, const float2* __restrict__ wn,
-                                           uint32_t batch, float scale, uint32_t g, float2* lds, int tid) {
+                                           const TWS& tws, uint32_t batch, float scale, uint32_t g,
+                                           float2* lds, int tid) {
     using PL = PlanOf<I, SC>;
     constexpr int M = PL::N, T = PL::T, B = WG / T, R0 = PL::R0, RL = PL::RL, P = PL::P;
     static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
@@ -239,11 +310,11 @@ __device__ __forceinline__ void c2r_finish(float2 (&q)[bfly_plan::kPoints + 1], 
             if (k == 0) { a.y = 0.f; b.y = 0.f; }
             const float2 s = make_float2(a.x + b.x, a.y - b.y);
             const float2 d = make_float2(a.x - b.x, a.y + b.y);
-            const float2 wd = cmul_tw<1>(d, wn[k]);  // W_n^{-k} d
+            const float2 wd = cmul_tw<1>(d, tws.w(k));  // W_n^{-k} d
             v[m * R0 + r] = make_float2(s.x - wd.y, s.y + wd.x);
         }
     __syncthreads();
-    fft_local<I, T, 1, SC>(v, L, tw, t);
+    tws.template fft<T, 1>(v, L, t);
     if (tr >= batch) return;
     float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M));
 #pragma unroll
@@ -262,7 +333,7 @@ __device__ __forceinline__ void c2r_unit(const float* __restrict__ in, float* __
                                          uint32_t g, float2* lds, int tid) {
     float2 q[bfly_plan::kPoints + 1];
     c2r_load<I, WG, SC>(q, in, batch, kin, istride, g, tid);
-    c2r_finish<I, WG, SC>(q, out, tw, wn, batch, scale, g, lds, tid);
+    c2r_finish<I, WG, SC>(q, out, RealTwGlobal<I, SC>{tw, wn}, batch, scale, g, lds, tid);
 }
 
 }  // namespace bfly_dev

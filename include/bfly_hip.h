@@ -176,9 +176,11 @@ BFLY_API int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch
 /* Real transforms with an explicit kernel variant (kind R2C or C2R; bins and
  * stride as for bfly_r2c_ex / bfly_c2r_ex): BFLY_VARIANT_PLAIN = one-shot
  * grid, BFLY_VARIANT_STREAM = resident grid that loads the next rows while
- * computing the current ones (same arithmetic, same bits), BFLY_VARIANT_AUTO
- * = the library's choice; other ids give BFLY_ERR_SHAPE.  Same reference
- * interfaces as bfly_r2c / bfly_c2r. */
+ * computing the current ones (same arithmetic, same bits),
+ * BFLY_VARIANT_STREAM_LT = streaming with the twiddles in LDS (two-level
+ * products; n >= 256, else STREAM), BFLY_VARIANT_AUTO = the library's
+ * choice; other ids give BFLY_ERR_SHAPE.  Same reference interfaces as
+ * bfly_r2c / bfly_c2r. */
 BFLY_API int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
                           uint32_t bins, uint32_t stride, int variant, const float* table,
                           void* stream);

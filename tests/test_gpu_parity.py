@@ -177,9 +177,40 @@ def test_real_streaming_variant_equals_one_shot(n):
         torch.cuda.synchronize()
         assert torch.equal(backs[0], backs[1]), (n, b, bins)
         assert oracle.rel_l2_rows(backs[1].cpu().numpy(), oracle.c2r_f64(h.cpu().numpy(), n)) <= REL_L2
-    # ids other than AUTO / PLAIN / STREAM are refused
+    # ids other than AUTO / PLAIN / STREAM / STREAM_LT are refused
     assert lib.bfly_real_ex(1, xt.data_ptr(), xt.data_ptr(), n, 1, n // 2 + 1, n + 2,
-                            _lib.VARIANT_STREAM_LT, tab.data_ptr(), s) == _lib.ERR_SHAPE
+                            _lib.VARIANT_STREAM2, tab.data_ptr(), s) == _lib.ERR_SHAPE
+
+
+@pytest.mark.parametrize("n", [256, 1024, 4096, 8192])
+def test_real_lds_twiddle_variant_fp64(n):
+    # bfly_real_ex STREAM_LT: twiddles from two-level LDS tables (different
+    # rounding from the one-shot kernel); FP64 gates, round trip, truncation.
+    import torch
+    from bfly import _lib
+    lib = _lib.hip()
+    s = torch.cuda.current_stream().cuda_stream
+    for b, bins in ((1, n // 2 + 1), (777, n // 2 + 1), (3000, 13)):
+        x = oracle.uniform_input(33, (b, n))
+        xt = torch.from_numpy(x).cuda()
+        y = torch.full((b, 2 * bins), float("nan"), device="cuda")
+        assert lib.bfly_real_ex(1, xt.data_ptr(), y.data_ptr(), n, b, bins, 2 * bins, _lib.VARIANT_STREAM_LT,
+                                _ops().table(1, n, xt.device).data_ptr(), s) == 0
+        torch.cuda.synchronize()
+        ref = oracle.r2c_f64(x, n)[:, :2 * bins]
+        assert oracle.rel_l2_rows(y.cpu().numpy(), np.ascontiguousarray(ref)) <= REL_L2
+        h = torch.zeros(b, n + 2, device="cuda")
+        h[:, :2 * bins] = y
+        h[:, 1] = 0
+        if bins == n // 2 + 1:
+            h[:, n + 1] = 0
+        z = torch.full((b, n), float("nan"), device="cuda")
+        assert lib.bfly_real_ex(2, h.data_ptr(), z.data_ptr(), n, b, bins, n + 2, _lib.VARIANT_STREAM_LT,
+                                _ops().table(2, n, xt.device).data_ptr(), s) == 0
+        torch.cuda.synchronize()
+        assert oracle.rel_l2_rows(z.cpu().numpy(), oracle.c2r_f64(h.cpu().numpy(), n)) <= REL_L2
+        if bins == n // 2 + 1:
+            assert np.max(np.abs(z.cpu().numpy() - x)) < 1e-5
 
 
 @pytest.mark.parametrize("n", [16, 32, 64, 1024, 4096])
