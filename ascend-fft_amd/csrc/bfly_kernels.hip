@@ -161,12 +161,21 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream2(
 // Streaming form with the two-level twiddle tables in LDS (no twiddle
 // registers): more workgroups stay resident, so more transforms are in
 // flight per CU.  `tw2g` points at [A (64) | B (N/64)] in the table.
+// Threads per workgroup of the LDS-twiddle streaming kernel: 256, or one
+// whole transform's T = n/16 above 4096 points.
+template <int I>
+constexpr int lt_wg() { return PlanOf<I>::T > bfly_plan::kBlock ? PlanOf<I>::T : bfly_plan::kBlock; }
+
 template <int I, int DIR>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
+__global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
     const float2* __restrict__ tw2g, uint32_t batch) {
     using PL = PlanOf<I>;
-    constexpr int N = PL::N, T = PL::T, B = bfly_plan::kBlock / T;
+    constexpr int WG = lt_wg<I>();
+    constexpr int N = PL::N, T = PL::T, B = WG / T;
+    // tw2g: [A (64) | B (N/64)] contiguous up to 4096 points; above, the full
+    // W_N^m table of the long layout, from which A = W_N^j and B = W_N^{64 j}
+    constexpr bool kFullTable = N > bfly_plan::kMaxLocalN;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
     constexpr int NLO = N < TW2_LO ? N : TW2_LO;
     constexpr int NHI = N / NLO;
@@ -178,9 +187,9 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_stream_lt(
     __shared__ float2 ltw[NLO + NHI + NT1];
     const int tid = threadIdx.x;
     const int bl = tid / T, t0 = tid % T;
-    for (int i = tid; i < NLO + NHI; i += bfly_plan::kBlock) ltw[i] = tw2g[i];
+    for (int i = tid; i < NLO + NHI; i += WG) ltw[i] = tw2g[(kFullTable && i >= NLO) ? (i - NLO) * NLO : i];
     if constexpr (PL::NRAD > 1)
-        for (int i = tid; i < NT1; i += bfly_plan::kBlock)
+        for (int i = tid; i < NT1; i += WG)
             ltw[NLO + NHI + (i % R1) * bfly_plan::kPoints + i / R1] = tw[i];
     __syncthreads();
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
@@ -479,11 +488,11 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
 
 // ------------------------------------------------------------ dispatch
 // Resident workgroups of kernel f on the current device (CUs x occupancy).
-int resident_blocks(const void* f) {
+int resident_blocks(const void* f, int block = bfly_plan::kBlock) {
     int dev = 0, cus = 0, per = 0;
     if (hipGetDevice(&dev) != hipSuccess ||
         hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, bfly_plan::kBlock, 0) != hipSuccess)
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, block, 0) != hipSuccess)
         return 0;
     return cus * (per > 0 ? per : 1);
 }
@@ -509,14 +518,13 @@ hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint3
 template <int I, int DIR>
 hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw, const float2* tw2g,
                          uint32_t batch, hipStream_t s) {
-    constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
+    constexpr int WG = lt_wg<I>(), B = WG / PlanOf<I>::T;
     const uint64_t groups = (batch + B - 1) / B;
     static int cap = 0;
-    if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>));
+    if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>), WG);
     if (cap <= 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)cap));
-    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, tw2g,
-                       batch);
+    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(WG), 0, s, in, out, tw, tw2g, batch);
     return hipGetLastError();
 }
 
@@ -662,6 +670,18 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
         default: break;
     }
     const bfly_host::TableLayout L = bfly_host::table_layout(0, n);
+    if (variant == BFLY_VARIANT_STREAM_LT && n <= (uint32_t)bfly_plan::kMaxSingleN) {
+        const float2 *ts = tw + L.single, *wN = tw + L.extra;
+        switch (n) {
+            case 8192:
+                return sign < 0 ? launch_c2c_lt<idx(8192), -1>(i2, o2, ts, wN, batch, s)
+                                : launch_c2c_lt<idx(8192), 1>(i2, o2, ts, wN, batch, s);
+            case 16384:
+                return sign < 0 ? launch_c2c_lt<idx(16384), -1>(i2, o2, ts, wN, batch, s)
+                                : launch_c2c_lt<idx(16384), 1>(i2, o2, ts, wN, batch, s);
+            default: break;
+        }
+    }
     if (variant != BFLY_VARIANT_FOUR_STEP) {
         switch (n) {
             case 8192: return launch_c2c_single<idx(8192)>(i2, o2, tw + L.single, batch, sign, s);

@@ -159,15 +159,17 @@ int check_execute(int kind, int sign, const void* in, const void* out, uint32_t 
 // streaming form with LDS two-level twiddles wins from n = 512 once the batch
 // fills several waves of resident workgroups (n=4096 b=4096: 50.4 us vs
 // 62.2 us one-shot); below that the one-shot grid is as fast or faster.
-// One workgroup per transform at 8192 / 16384 beats the four-step chain once
-// the batch fills the compute units (8192 b=2048: 70.6 vs 111.4 us; 16384
-// b=1024: 82.0 vs 113.7 us); with few transforms the four-step chain's wider
-// grid wins (16384 b=64: 12.5 vs 17.0 us; 8192 b=64: a tie).
+// One workgroup per transform at 8192 / 16384 (its LDS-twiddle streaming
+// form) beats the four-step chain once the batch fills the compute units
+// (8192 b=2048: 56.5 vs 93.3 us; 8192 b=64: 9.4 vs 11.9 us; 16384 b=1024:
+// 74.6 vs 98.1 us; 16384 b=256: 19.4 vs 31.1 us); with few 16384-point
+// transforms the four-step chain's wider grid wins (b=64: 11.6 vs 13.8 us).
 int auto_variant(uint32_t n, uint32_t batch) {
     const uint64_t points = (uint64_t)n * batch;
     if (n >= 512 && n <= (uint32_t)bfly_plan::kMaxLocalN && points >= (1ull << 22))
         return BFLY_VARIANT_STREAM_LT;
-    if ((n == 8192 && batch < 64) || (n == 16384 && batch < 256)) return BFLY_VARIANT_FOUR_STEP;
+    if (n == 8192) return BFLY_VARIANT_STREAM_LT;
+    if (n == 16384) return batch < 256 ? BFLY_VARIANT_FOUR_STEP : BFLY_VARIANT_STREAM_LT;
     return BFLY_VARIANT_PLAIN;
 }
 

@@ -141,11 +141,20 @@ def secondary_rows(cpu_seconds: float, reps: int = 20):
                 "transforms_per_s": round(transforms / (ms * 1e-3), 1), "algorithmic_bytes": int(nbytes)}
 
     out = {}
-    # cfg1: C2C n=1024 b=64 (latency-bound)
+    # cfg1: C2C n=1024 b=64 (latency-bound): back-to-back launches from C
+    # (bfly_rt_time_transform, the Plan::measure loop), so Python call
+    # overhead does not set the rate
+    import ctypes
     n, b = 1024, 64
-    xs = [torch.rand(b, 2 * n, device=dev) for _ in range(4)]
-    ys = [torch.empty_like(xs[0]) for _ in range(4)]
-    out["c2c_n1024_b64"] = row(timed(lambda i: ops.c2c_raw(xs[i % 4], ys[i % 4], n, b, -1)), 16 * n * b, b)
+    x1 = torch.rand(b, 2 * n, device=dev)
+    y1 = torch.empty_like(x1)
+    tab1 = ops.table(_lib.KIND_C2C, n, dev)
+    ms1 = ctypes.c_float()
+    for _ in range(2):
+        _lib.check(lib.bfly_rt_time_transform(_lib.KIND_C2C, -1, x1.data_ptr(), y1.data_ptr(), n, b,
+                                              tab1.data_ptr(), None, 200, sid(), ctypes.byref(ms1)),
+                   "bfly_rt_time_transform")
+    out["c2c_n1024_b64"] = row(ms1.value / 200, 16 * n * b, b)
     # R2C / C2R 4096 x 4096
     n, b = 4096, 4096
     xr = [torch.rand(b, n, device=dev) * 2 - 1 for _ in range(4)]

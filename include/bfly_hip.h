@@ -86,11 +86,10 @@ BFLY_API size_t bfly_table_floats(int kind, uint32_t n);
 BFLY_API int bfly_table_fill(int kind, uint32_t n, float* host_table);
 
 /* Device workspace (bytes) an execute call with BFLY_VARIANT_AUTO needs; 0
- * for every shape that runs in one kernel (C2C up to 4096, 8192 with
- * batch >= 64, 16384 with batch >= 256; all real transforms).  The four-step
- * path (C2C above 16384, the smaller batches of 8192 / 16384, and
- * BFLY_VARIANT_FOUR_STEP) needs n*batch complex values for the
- * intermediate. */
+ * for every shape that runs in one kernel (C2C up to 8192, 16384 with
+ * batch >= 256; all real transforms).  The four-step path (C2C above 16384,
+ * 16384 with batch < 256, and BFLY_VARIANT_FOUR_STEP) needs n*batch complex
+ * values for the intermediate. */
 BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
 
 /* --------------------------------------------------------------- execute
@@ -124,6 +123,8 @@ BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, i
  *   BFLY_VARIANT_FOUR_STEP  n = 8192 / 16384 through the two-kernel four-step
  *                        path (needs n*batch*8 bytes of workspace) instead of
  *                        one 512 / 1024-thread workgroup per transform
+ *                        (PLAIN: one-shot grid; STREAM_LT: resident
+ *                        streaming grid with LDS twiddles)
  *   BFLY_VARIANT_SPLIT_MFMA n = 1024 / 2048 / 4096 only (else BFLY_ERR_SHAPE):
  *                        the radix-16 passes as 16x16 complex DFT products on
  *                        the matrix cores in split FP16 (hi + lo operands,
@@ -132,9 +133,9 @@ BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, i
  *                        about 3x that of the FP32 variants, within 1e-5
  *   BFLY_VARIANT_SPLIT_MFMA_STREAM  the same on a resident grid that
  *                        prefetches each workgroup's next transforms
- * For n <= 4096 FOUR_STEP means PLAIN; above 4096 every other id selects the
- * length's default path (one workgroup per transform up to 16384, four-step
- * above). */
+ * For n <= 4096 FOUR_STEP means PLAIN.  At 8192 / 16384, PLAIN and
+ * STREAM_LT run one workgroup per transform (one-shot / streaming) and the
+ * other ids the one-shot form; above 16384 every id runs four-step. */
 #define BFLY_VARIANT_AUTO 0
 #define BFLY_VARIANT_PLAIN 1
 #define BFLY_VARIANT_STREAM 2

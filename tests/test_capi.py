@@ -103,12 +103,12 @@ def test_long_table_layout(n):
     tail = _stage_table(single).astype(np.complex64) if single else np.zeros(0, np.complex64)
     assert np.array_equal(c[o + n:], tail)
     # the default path needs the n*batch workspace only where it runs four-step:
-    # above 16384, and below the batch that fills the GPU one transform per
-    # workgroup (8192: 64, 16384: 256)
+    # above 16384, and 16384 below the batch that fills the GPU one transform
+    # per workgroup (256); 8192 always runs one workgroup per transform
     assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 64) == (0 if n == 8192 else n * 64 * 8)
     if n <= 16384:
         assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 256) == 0
-        assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 63) == n * 63 * 8
+        assert lib.bfly_workspace_bytes(bfly.KIND_C2C, n, 63) == (0 if n == 8192 else n * 63 * 8)
 
 
 @pytest.mark.parametrize("kind", [bfly.KIND_R2C, bfly.KIND_C2R])
@@ -151,7 +151,7 @@ def test_execute_guards_through_stub():
     assert s.bfly_c2c(p, p, 32768, 1, -1, p, None, None) == bfly.ERR_SHAPE  # four-step needs work
     assert s.bfly_c2c(p, p, 32768, 1, -1, p, p, None) == 0
     assert s.bfly_c2c(p, p, 8192, 64, -1, p, None, None) == 0  # one workgroup per transform
-    assert s.bfly_c2c(p, p, 8192, 1, -1, p, None, None) == bfly.ERR_SHAPE  # small batch: four-step
+    assert s.bfly_c2c(p, p, 16384, 1, -1, p, None, None) == bfly.ERR_SHAPE  # small batch: four-step
     assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_PLAIN, p, None, None) == 0
     assert s.bfly_c2c_ex(p, p, 8192, 64, -1, _lib.VARIANT_FOUR_STEP, p, None, None) == bfly.ERR_SHAPE
     assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_LAST + 1, p, None, None) == bfly.ERR_SHAPE

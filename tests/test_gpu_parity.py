@@ -232,7 +232,7 @@ def test_long_c2c_b64_seed2(n):
 
 
 @pytest.mark.parametrize("n", [8192, 16384])
-@pytest.mark.parametrize("b", [1, 5, 64])
+@pytest.mark.parametrize("b", [1, 5, 64, 700])
 def test_single_workgroup_and_four_step_paths_agree(n, b):
     # n = 8192 / 16384 as one 512 / 1024-thread workgroup per transform
     # (BFLY_VARIANT_PLAIN, no workspace) and as the two-kernel four-step path
@@ -247,7 +247,7 @@ def test_single_workgroup_and_four_step_paths_agree(n, b):
     work = torch.empty(n * b * 2, dtype=torch.float32, device="cuda")
     outs = []
     for variant, w in ((_lib.VARIANT_PLAIN, None), (_lib.VARIANT_FOUR_STEP, work.data_ptr()),
-                       (_lib.VARIANT_AUTO, work.data_ptr())):
+                       (_lib.VARIANT_STREAM_LT, None), (_lib.VARIANT_AUTO, work.data_ptr())):
         y = torch.empty_like(t)
         rc = _lib.hip().bfly_c2c_ex(t.data_ptr(), y.data_ptr(), n, b, -1, variant, tab.data_ptr(),
                                     w, torch.cuda.current_stream().cuda_stream)
