@@ -156,9 +156,10 @@ int check_execute(int kind, int sign, const void* in, const void* out, uint32_t 
 }
 
 // Measured on MI355X (C2C, 4 rotated buffer pairs, HIP events): the
-// streaming form with LDS two-level twiddles wins from n = 512 once the batch
-// fills several waves of resident workgroups (n=4096 b=4096: 50.4 us vs
-// 62.2 us one-shot); below that the one-shot grid is as fast or faster.
+// streaming form with LDS twiddles wins from n = 256 once the batch fills
+// several waves of resident workgroups (n=4096 b=4096: 47.5 us vs 62.2 us
+// one-shot; n=256 b=65536: 46.1 vs 48.5 us); below that the one-shot grid
+// is as fast or faster (n=128 b=131072: 49.4 vs 50.3 us).
 // One workgroup per transform at 8192 / 16384 (its LDS-twiddle streaming
 // form) beats the four-step chain once the batch fills the compute units
 // (8192 b=2048: 56.5 vs 93.3 us; 8192 b=64: 9.4 vs 11.9 us; 16384 b=1024:
@@ -166,7 +167,7 @@ int check_execute(int kind, int sign, const void* in, const void* out, uint32_t 
 // transforms the four-step chain's wider grid wins (b=64: 11.6 vs 13.8 us).
 int auto_variant(uint32_t n, uint32_t batch) {
     const uint64_t points = (uint64_t)n * batch;
-    if (n >= 512 && n <= (uint32_t)bfly_plan::kMaxLocalN && points >= (1ull << 22))
+    if (n >= 256 && n <= (uint32_t)bfly_plan::kMaxLocalN && points >= (1ull << 22))
         return BFLY_VARIANT_STREAM_LT;
     if (n == 8192) return BFLY_VARIANT_STREAM_LT;
     if (n == 16384) return batch < 256 ? BFLY_VARIANT_FOUR_STEP : BFLY_VARIANT_STREAM_LT;
