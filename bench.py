@@ -56,7 +56,7 @@ def parse():
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="CPU baseline budget")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--only", default=None,
-                    choices=["grouped", "r2c", "c2r", "mfma", "long65536", "single16384"],
+                    choices=["grouped", "r2c", "c2r", "mfma", "long65536", "single16384", "spectral"],
                     help="run one secondary workload loop instead of the headline (profiling)")
     ap.add_argument("--no-patterns", action="store_true",
                     help="skip the grouped uneven-batch measurement")
@@ -332,7 +332,20 @@ def workload_loop(name: str, steps: int, warmup: int):
     def rand(*shape):
         return torch.rand(*shape, generator=g, device=dev) * 2 - 1
 
-    if name == "grouped":
+    if name == "spectral":
+        from bfly.spectral import spectral_conv1d
+        torch.manual_seed(4)
+        B, Cc, Nn, K = 64, 128, 4096, 256
+        xs = torch.randn(B, Cc, Nn, device=dev)
+        sc = (1.0 / (Cc * 2)) ** 0.5
+        ws = torch.complex(torch.randn(Cc, Cc, K, device=dev) * sc, torch.randn(Cc, Cc, K, device=dev) * sc)
+        up = torch.randn(B, Cc, Nn, device=dev)
+
+        def fn():
+            xa, wa = xs.clone().requires_grad_(True), ws.clone().requires_grad_(True)
+            (spectral_conv1d(xa, wa) * up).sum().backward()
+        nbytes = 4 * (2 * B * Cc * Nn) + 8 * Cc * Cc * K
+    elif name == "grouped":
         reqs = group.make_requests(group.uneven_specs(), seed=3)
         grp = group.Group(reqs, group.Config(dynamic=1))
         graph = group.capture(grp.launch)
