@@ -37,6 +37,9 @@ VARIANT_SPLIT_MFMA = 8
 VARIANT_SPLIT_MFMA_STREAM = 9
 VARIANT_LAST = 9
 
+REAL_ADJOINT = 1  # bfly_real_ex flag: the transform's adjoint
+MIX_FORWARD, MIX_GRAD_INPUT, MIX_GRAD_WEIGHT = 0, 1, 2  # bfly_mode_mix ops
+
 KIND_C2C = 0
 KIND_R2C = 1
 KIND_C2R = 2
@@ -59,7 +62,7 @@ SYMBOLS = (
     "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
     "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
-    "bfly_group_query", "bfly_group_destroy", "bfly_probe_copy", "bfly_probe_device",
+    "bfly_group_query", "bfly_group_destroy", "bfly_mode_mix", "bfly_probe_copy", "bfly_probe_device",
 )
 
 
@@ -104,7 +107,8 @@ def _bind(lib):
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_r2c_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
         "bfly_c2r_ex": (i32, [vp, vp, u32, u32, u32, u32, vp, vp]),
-        "bfly_real_ex": (i32, [i32, vp, vp, u32, u32, u32, u32, i32, vp, vp]),
+        "bfly_real_ex": (i32, [i32, vp, vp, u32, u32, u32, u32, i32, u32, vp, vp]),
+        "bfly_mode_mix": (i32, [i32, vp, vp, vp, u32, u32, u32, u32, vp]),
         "bfly_group_supported": (i32, [i32, u32, i32]),
         "bfly_group_create": (i32, [C.POINTER(GroupRequest), i32, C.POINTER(GroupConfig),
                                     C.POINTER(vp)]),

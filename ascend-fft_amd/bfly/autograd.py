@@ -6,7 +6,9 @@ Adjoints (unnormalised forward F, conjugate-Wirtinger gradients as torch uses):
                                        1/2 elsewhere (the Hermitian weights of the real input)
   irfft(h, n)   x = irfft(h)           grad_h = rfft(grad_x) * v / n, v = 1 at bins 0 and n/2,
                                        2 elsewhere, imaginary parts of bins 0 and n/2 zero
-Every adjoint is itself one launch of the library's kernels (no torch.fft call).
+Every adjoint is one launch of the library's kernels with the weighting done
+inside the kernel (bfly_real_ex with BFLY_REAL_ADJOINT; no torch.fft call, no
+separate weighting pass).
 """
 from __future__ import annotations
 
@@ -39,19 +41,12 @@ class RFFT(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, x: torch.Tensor):
-        n = x.shape[-1]
-        ctx.n = n
-        h = ops.rfft(x.contiguous())
-        return _as_complex(h)
+        ctx.n = x.shape[-1]
+        return _as_complex(ops.rfft(x.contiguous()))
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        n = ctx.n
-        w = torch.full((n // 2 + 1,), 0.5, dtype=torch.float32, device=g.device)
-        w[0] = 1.0
-        w[-1] = 1.0
-        gw = _as_interleaved(g * w)
-        return ops.irfft(gw, n) * n
+        return ops.irfft(_as_interleaved(g), ctx.n, adjoint=True)
 
 
 class IRFFT(torch.autograd.Function):
@@ -64,24 +59,7 @@ class IRFFT(torch.autograd.Function):
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        n = ctx.n
-        z = _as_complex(ops.rfft(g.contiguous()))
-        v = torch.full((n // 2 + 1,), 2.0 / n, dtype=torch.float32, device=g.device)
-        v[0] = 1.0 / n
-        v[-1] = 1.0 / n
-        z = z * v
-        # Bins 0 and n/2 of irfft only read their real parts.
-        z[..., 0] = z[..., 0].real.to(z.dtype)
-        z[..., -1] = z[..., -1].real.to(z.dtype)
-        return z, None
-
-
-def _modes_weights(n: int, modes: int, interior: float, edge: float, device) -> torch.Tensor:
-    w = torch.full((modes,), interior, dtype=torch.float32, device=device)
-    w[0] = edge
-    if modes == n // 2 + 1:
-        w[-1] = edge
-    return w
+        return _as_complex(ops.rfft(g.contiguous(), adjoint=True)), None
 
 
 class RFFTModes(torch.autograd.Function):
@@ -89,14 +67,12 @@ class RFFTModes(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, x: torch.Tensor, modes: int):
-        ctx.n, ctx.modes = x.shape[-1], modes
+        ctx.n = x.shape[-1]
         return _as_complex(ops.rfft_modes(x.contiguous(), modes))
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        n, K = ctx.n, ctx.modes
-        gw = _as_interleaved(g * _modes_weights(n, K, 0.5, 1.0, g.device))
-        return ops.irfft_modes(gw, n) * n, None
+        return ops.irfft_modes(_as_interleaved(g), ctx.n, adjoint=True), None
 
 
 class IRFFTModes(torch.autograd.Function):
@@ -104,18 +80,12 @@ class IRFFTModes(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, h: torch.Tensor, n: int):
-        ctx.n, ctx.modes = n, h.shape[-1]
+        ctx.modes = h.shape[-1]
         return ops.irfft_modes(_as_interleaved(h), n)
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        n, K = ctx.n, ctx.modes
-        z = _as_complex(ops.rfft_modes(g.contiguous(), K))
-        z = z * _modes_weights(n, K, 2.0 / n, 1.0 / n, g.device)
-        z[..., 0] = z[..., 0].real.to(z.dtype)
-        if K == n // 2 + 1:
-            z[..., -1] = z[..., -1].real.to(z.dtype)
-        return z, None
+        return _as_complex(ops.rfft_modes(g.contiguous(), ctx.modes, adjoint=True)), None
 
 
 def rfft_modes(x: torch.Tensor, modes: int) -> torch.Tensor:

@@ -88,8 +88,20 @@ def c2c(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
     return out
 
 
-def rfft(x: torch.Tensor) -> torch.Tensor:
-    """R2C along the last axis: float32 [..., n] -> half spectrum [..., n+2]."""
+def _real(kind: int, src: torch.Tensor, dst: torch.Tensor, n: int, batch: int, bins: int, stride: int,
+          adjoint: bool, what: str) -> None:
+    tab = table(kind, n, src.device)
+    rc = _lib.hip().bfly_real_ex(kind, src.data_ptr(), dst.data_ptr(), n, batch, bins, stride,
+                                 _lib.VARIANT_AUTO, _lib.REAL_ADJOINT if adjoint else 0, tab.data_ptr(),
+                                 _stream(src.device))
+    _lib.check(rc, f"{what}(n={n}, batch={batch}, bins={bins}, adjoint={adjoint})")
+
+
+def rfft(x: torch.Tensor, adjoint: bool = False) -> torch.Tensor:
+    """R2C along the last axis: float32 [..., n] -> half spectrum [..., n+2].
+
+    adjoint=True returns the adjoint of irfft instead: bin k times v_k / n,
+    v = 1 at bins 0 and n/2 (imaginary part 0) and 2 elsewhere."""
     _check_device(x, "x")
     if x.dtype != torch.float32:
         raise TypeError("rfft takes float32")
@@ -98,15 +110,15 @@ def rfft(x: torch.Tensor) -> torch.Tensor:
     out = torch.empty(*x.shape[:-1], n + 2, dtype=torch.float32, device=x.device)
     if batch == 0:
         return out
-    tab = table(_lib.KIND_R2C, n, x.device)
-    rc = _lib.hip().bfly_r2c(x.data_ptr(), out.data_ptr(), n, batch, tab.data_ptr(),
-                             _stream(x.device))
-    _lib.check(rc, f"bfly_r2c(n={n}, batch={batch})")
+    _real(_lib.KIND_R2C, x, out, n, batch, n // 2 + 1, n + 2, adjoint, "bfly_r2c")
     return out
 
 
-def irfft(h: torch.Tensor, n: int) -> torch.Tensor:
-    """C2R along the last axis: half spectrum [..., n+2] -> float32 [..., n], 1/n."""
+def irfft(h: torch.Tensor, n: int, adjoint: bool = False) -> torch.Tensor:
+    """C2R along the last axis: half spectrum [..., n+2] -> float32 [..., n], 1/n.
+
+    adjoint=True returns the adjoint of rfft instead: n * irfft(w h), w = 1
+    at bins 0 and n/2 and 1/2 elsewhere."""
     _check_device(h, "h")
     if h.dtype != torch.float32 or h.shape[-1] != n + 2:
         raise ValueError("irfft takes float32 [..., n+2]")
@@ -114,42 +126,56 @@ def irfft(h: torch.Tensor, n: int) -> torch.Tensor:
     out = torch.empty(*h.shape[:-1], n, dtype=torch.float32, device=h.device)
     if batch == 0:
         return out
-    tab = table(_lib.KIND_C2R, n, h.device)
-    rc = _lib.hip().bfly_c2r(h.data_ptr(), out.data_ptr(), n, batch, tab.data_ptr(),
-                             _stream(h.device))
-    _lib.check(rc, f"bfly_c2r(n={n}, batch={batch})")
+    _real(_lib.KIND_C2R, h, out, n, batch, n // 2 + 1, n + 2, adjoint, "bfly_c2r")
     return out
 
 
-def rfft_modes(x: torch.Tensor, modes: int) -> torch.Tensor:
+def rfft_modes(x: torch.Tensor, modes: int, adjoint: bool = False) -> torch.Tensor:
     """First `modes` bins of the R2C along the last axis: float32 [..., n] ->
-    float32 [..., 2*modes] (interleaved); the other bins are never written."""
+    float32 [..., 2*modes] (interleaved); the other bins are never written.
+    adjoint=True: the adjoint of irfft_modes (weights as for rfft)."""
     _check_device(x, "x")
     n = x.shape[-1]
     batch = x.numel() // n if n else 0
     out = torch.empty(*x.shape[:-1], 2 * modes, dtype=torch.float32, device=x.device)
     if batch == 0:
         return out
-    tab = table(_lib.KIND_R2C, n, x.device)
-    rc = _lib.hip().bfly_r2c_ex(x.data_ptr(), out.data_ptr(), n, batch, modes, 2 * modes,
-                                tab.data_ptr(), _stream(x.device))
-    _lib.check(rc, f"bfly_r2c_ex(n={n}, batch={batch}, modes={modes})")
+    _real(_lib.KIND_R2C, x, out, n, batch, modes, 2 * modes, adjoint, "bfly_r2c_modes")
     return out
 
 
-def irfft_modes(h: torch.Tensor, n: int) -> torch.Tensor:
+def irfft_modes(h: torch.Tensor, n: int, adjoint: bool = False) -> torch.Tensor:
     """C2R of a spectrum given by its first modes bins (float32 [..., 2*modes],
-    interleaved; every higher bin zero) -> float32 [..., n], 1/n."""
+    interleaved; every higher bin zero) -> float32 [..., n], 1/n.
+    adjoint=True: the adjoint of rfft_modes (weights as for irfft)."""
     _check_device(h, "h")
     modes = h.shape[-1] // 2
     batch = h.numel() // (2 * modes)
     out = torch.empty(*h.shape[:-1], n, dtype=torch.float32, device=h.device)
     if batch == 0:
         return out
-    tab = table(_lib.KIND_C2R, n, h.device)
-    rc = _lib.hip().bfly_c2r_ex(h.data_ptr(), out.data_ptr(), n, batch, modes, 2 * modes,
-                                tab.data_ptr(), _stream(h.device))
-    _lib.check(rc, f"bfly_c2r_ex(n={n}, batch={batch}, modes={modes})")
+    _real(_lib.KIND_C2R, h, out, n, batch, modes, 2 * modes, adjoint, "bfly_c2r_modes")
+    return out
+
+
+def mode_mix(op: int, a: torch.Tensor, b: torch.Tensor, batch: int, c_in: int, c_out: int,
+             modes: int) -> torch.Tensor:
+    """bfly_mode_mix on complex64 tensors (mode index last, contiguous):
+    op MIX_FORWARD   a = x [B, C_in, K],  b = w [C_in, C_out, K]  -> y  [B, C_out, K]
+    op MIX_GRAD_INPUT  a = gy [B, C_out, K], b = w             -> gx [B, C_in, K]
+    op MIX_GRAD_WEIGHT a = x [B, C_in, K], b = gy [B, C_out, K] -> gw [C_in, C_out, K]"""
+    for t, name in ((a, "a"), (b, "b")):
+        _check_device(t, name)
+        if t.dtype != torch.complex64:
+            raise TypeError(f"mode_mix takes complex64 ({name} is {t.dtype})")
+    shape = {_lib.MIX_FORWARD: (batch, c_out, modes), _lib.MIX_GRAD_INPUT: (batch, c_in, modes),
+             _lib.MIX_GRAD_WEIGHT: (c_in, c_out, modes)}.get(op)
+    if shape is None:
+        raise ValueError(f"unknown mode_mix op {op}")
+    out = torch.empty(shape, dtype=torch.complex64, device=a.device)
+    rc = _lib.hip().bfly_mode_mix(op, a.data_ptr(), b.data_ptr(), out.data_ptr(), batch, c_in, c_out, modes,
+                                  _stream(a.device))
+    _lib.check(rc, f"bfly_mode_mix(op={op}, batch={batch}, c_in={c_in}, c_out={c_out}, modes={modes})")
     return out
 
 

@@ -89,20 +89,22 @@ int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
 }
 
 int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                 uint32_t stride, int variant, const float* table, void* stream) {
+                 uint32_t stride, int variant, uint32_t flags, const float* table, void* stream) {
     if (kind != BFLY_KIND_R2C && kind != BFLY_KIND_C2R) return BFLY_ERR_SHAPE;
+    if (flags & ~BFLY_REAL_ADJOINT) return BFLY_ERR_SHAPE;
     const int sign = kind == BFLY_KIND_R2C ? -1 : 1;
     if (int g = bfly_host::check_execute(kind, sign, in, out, n, batch, table, nullptr)) return g;
     if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
     if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_real_variant(n, batch);
     if (variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM && variant != BFLY_VARIANT_STREAM_LT)
         return BFLY_ERR_SHAPE;
-    return launched(bfly_kern::real(kind, in, out, n, batch, bins, stride, variant, table, S(stream)));
+    return launched(bfly_kern::real(kind, in, out, n, batch, bins, stride, variant,
+                                    (flags & BFLY_REAL_ADJOINT) != 0, table, S(stream)));
 }
 
 int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                 uint32_t out_stride, const float* table, void* stream) {
-    return bfly_real_ex(BFLY_KIND_R2C, in, out, n, batch, bins, out_stride, BFLY_VARIANT_AUTO, table, stream);
+    return bfly_real_ex(BFLY_KIND_R2C, in, out, n, batch, bins, out_stride, BFLY_VARIANT_AUTO, 0, table, stream);
 }
 
 int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
@@ -112,7 +114,7 @@ int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const floa
 
 int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                 uint32_t in_stride, const float* table, void* stream) {
-    return bfly_real_ex(BFLY_KIND_C2R, in, out, n, batch, bins, in_stride, BFLY_VARIANT_AUTO, table, stream);
+    return bfly_real_ex(BFLY_KIND_C2R, in, out, n, batch, bins, in_stride, BFLY_VARIANT_AUTO, 0, table, stream);
 }
 
 int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
@@ -143,6 +145,12 @@ int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
     return rc;
+}
+
+int bfly_mode_mix(int op, const float* a, const float* b, float* c, uint32_t batch, uint32_t c_in,
+                  uint32_t c_out, uint32_t modes, void* stream) {
+    if (int e = bfly_host::check_mode_mix(op, a, b, c, batch, c_in, c_out, modes)) return e;
+    return launched(bfly_kern::mode_mix(op, a, b, c, batch, c_in, c_out, modes, S(stream)));
 }
 
 int bfly_probe_copy(const void* src, void* dst, size_t bytes, void* stream) {

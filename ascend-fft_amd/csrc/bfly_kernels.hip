@@ -243,10 +243,13 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c(const float* __restri
                                                            const float2* __restrict__ tw,
                                                            const float2* __restrict__ wn,
                                                            uint32_t batch, uint32_t kout,
-                                                           uint32_t ostride) {
+                                                           uint32_t ostride, int adj) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
     __shared__ float2 lds[B * lds_len(M)];
-    r2c_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, kout, ostride, blockIdx.x, lds, threadIdx.x);
+    float2 v[bfly_plan::kPoints];
+    r2c_load<I, bfly_plan::kBlock>(v, in, batch, blockIdx.x, threadIdx.x);
+    r2c_finish<I, bfly_plan::kBlock>(v, out, RealTwGlobal<I, 0>{tw, wn}, batch, kout, ostride, blockIdx.x, lds,
+                                     threadIdx.x, adj != 0);
 }
 
 // R2C, streaming form: a grid sized to the resident capacity walks the
@@ -257,7 +260,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream(const float* _
                                                                   const float2* __restrict__ tw,
                                                                   const float2* __restrict__ wn,
                                                                   uint32_t batch, uint32_t kout,
-                                                                  uint32_t ostride) {
+                                                                  uint32_t ostride, int adj) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
     __shared__ float2 lds[B * lds_len(M)];
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
@@ -273,7 +276,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream(const float* _
         int t2 = tid;
         asm volatile("" : "+v"(t2));
         if (gn < groups) r2c_load<I, bfly_plan::kBlock>(nxt, in, batch, gn, t2);
-        r2c_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, kout, ostride, g, lds, t2);
+        r2c_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, kout, ostride, g, lds, t2,
+                                         adj != 0);
         __syncthreads();  // the post-twiddle reads of `lds` end before the next FFT writes it
 #pragma unroll
         for (int i = 0; i < P; i++) cur[i] = nxt[i];
@@ -291,10 +295,13 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r(const float* __restri
                                                            const float2* __restrict__ tw,
                                                            const float2* __restrict__ wn,
                                                            uint32_t batch, float scale, uint32_t kin,
-                                                           uint32_t istride) {
+                                                           uint32_t istride, int adj) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T;
     __shared__ float2 lds[B * lds_len(M)];
-    c2r_unit<I, bfly_plan::kBlock>(in, out, tw, wn, batch, scale, kin, istride, blockIdx.x, lds, threadIdx.x);
+    float2 q[bfly_plan::kPoints + 1];
+    c2r_load<I, bfly_plan::kBlock>(q, in, batch, kin, istride, blockIdx.x, threadIdx.x);
+    c2r_finish<I, bfly_plan::kBlock>(q, out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, blockIdx.x, lds, threadIdx.x,
+                                     adj != 0);
 }
 
 // C2R, streaming form (see k_r2c_stream).
@@ -304,7 +311,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream(const float* _
                                                                   const float2* __restrict__ tw,
                                                                   const float2* __restrict__ wn,
                                                                   uint32_t batch, float scale, uint32_t kin,
-                                                                  uint32_t istride) {
+                                                                  uint32_t istride, int adj) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
     __shared__ float2 lds[B * lds_len(M)];
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
@@ -317,7 +324,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream(const float* _
         int t2 = tid;  // opaque per iteration (see k_r2c_stream)
         asm volatile("" : "+v"(t2));
         if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
-        c2r_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, g, lds, t2);
+        c2r_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, g, lds, t2, adj != 0);
         __syncthreads();  // the last exchange's reads end before the next staging writes
 #pragma unroll
         for (int i = 0; i <= P; i++) cur[i] = nxt[i];
@@ -333,7 +340,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream_lt(const float
                                                                      const float2* __restrict__ tw,
                                                                      const float2* __restrict__ wn,
                                                                      uint32_t batch, uint32_t kout,
-                                                                     uint32_t ostride) {
+                                                                     uint32_t ostride, int adj) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
     __shared__ float2 lds[B * lds_len(M)];
     __shared__ float2 ltw[real_lds_tw_len<I>()];
@@ -349,7 +356,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream_lt(const float
         int t2 = tid;  // opaque per iteration (see k_r2c_stream)
         asm volatile("" : "+v"(t2));
         if (gn < groups) r2c_load<I, bfly_plan::kBlock>(nxt, in, batch, gn, t2);
-        r2c_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, kout, ostride, g, lds, t2);
+        r2c_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, kout, ostride, g, lds, t2, adj != 0);
         __syncthreads();
 #pragma unroll
         for (int i = 0; i < P; i++) cur[i] = nxt[i];
@@ -362,7 +369,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float
                                                                      const float2* __restrict__ tw,
                                                                      const float2* __restrict__ wn,
                                                                      uint32_t batch, float scale,
-                                                                     uint32_t kin, uint32_t istride) {
+                                                                     uint32_t kin, uint32_t istride, int adj) {
     constexpr int M = PlanOf<I>::N, B = bfly_plan::kBlock / PlanOf<I>::T, P = bfly_plan::kPoints;
     __shared__ float2 lds[B * lds_len(M)];
     __shared__ float2 ltw[real_lds_tw_len<I>()];
@@ -378,7 +385,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float
         int t2 = tid;  // opaque per iteration (see k_r2c_stream)
         asm volatile("" : "+v"(t2));
         if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
-        c2r_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, scale, g, lds, t2);
+        c2r_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, scale, g, lds, t2, adj != 0);
         __syncthreads();
 #pragma unroll
         for (int i = 0; i <= P; i++) cur[i] = nxt[i];
@@ -563,9 +570,11 @@ hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uin
 
 template <int I>
 hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, const float2* wn,
-                       uint32_t batch, uint32_t bins, uint32_t stride, int variant, hipStream_t s) {
+                       uint32_t batch, uint32_t bins, uint32_t stride, int variant, int adj, hipStream_t s) {
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
     const uint64_t groups = (batch + B - 1) / B;
+    // C2R scale: 1/n, or 1/2 for the adjoint (its k = 0 term is doubled in the kernel)
+    const float scale = adj ? 0.5f : 1.0f / (float)(2 * PlanOf<I>::N);
     if constexpr (PlanOf<I>::N >= 128) {
         if (variant == BFLY_VARIANT_STREAM_LT) {
             const void* f = kind == 1 ? reinterpret_cast<const void*>(&k_r2c_stream_lt<I>)
@@ -577,10 +586,10 @@ hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, 
             const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)c));
             if (kind == 1)
                 hipLaunchKernelGGL((k_r2c_stream_lt<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
-                                   batch, bins, stride);
+                                   batch, bins, stride, adj);
             else
                 hipLaunchKernelGGL((k_c2r_stream_lt<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
-                                   batch, 1.0f / (float)(2 * PlanOf<I>::N), bins, stride);
+                                   batch, scale, bins, stride, adj);
             return hipGetLastError();
         }
     }
@@ -595,19 +604,19 @@ hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, 
         const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
         if (kind == 1)
             hipLaunchKernelGGL((k_r2c_stream<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
-                               batch, bins, stride);
+                               batch, bins, stride, adj);
         else
             hipLaunchKernelGGL((k_c2r_stream<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn,
-                               batch, 1.0f / (float)(2 * PlanOf<I>::N), bins, stride);
+                               batch, scale, bins, stride, adj);
         return hipGetLastError();
     }
     const dim3 grid((unsigned)groups);
     if (kind == 1)
         hipLaunchKernelGGL((k_r2c<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch,
-                           bins, stride);
+                           bins, stride, adj);
     else
         hipLaunchKernelGGL((k_c2r<I>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, wn, batch,
-                           1.0f / (float)(2 * PlanOf<I>::N), bins, stride);
+                           scale, bins, stride, adj);
     return hipGetLastError();
 }
 
@@ -701,20 +710,21 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
 }
 
 hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                uint32_t stride, int variant, const float* table, hipStream_t s) {
+                uint32_t stride, int variant, bool adjoint, const float* table, hipStream_t s) {
+    const int adj = adjoint ? 1 : 0;
     const float2* tw = reinterpret_cast<const float2*>(table);
     const bfly_host::TableLayout L = bfly_host::table_layout(kind, n);
     const float2* wn = tw + L.extra;
     switch (n / 2) {
-        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
-        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, bins, stride, variant, s);
+        case 16: return launch_real<idx(16)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 32: return launch_real<idx(32)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 64: return launch_real<idx(64)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 128: return launch_real<idx(128)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 256: return launch_real<idx(256)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 512: return launch_real<idx(512)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 1024: return launch_real<idx(1024)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 2048: return launch_real<idx(2048)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
+        case 4096: return launch_real<idx(4096)>(kind, in, out, tw, wn, batch, bins, stride, variant, adj, s);
         default: return hipErrorInvalidValue;
     }
 }

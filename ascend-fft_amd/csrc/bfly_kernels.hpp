@@ -8,10 +8,16 @@ namespace bfly_kern {
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s);
-// variant: BFLY_VARIANT_PLAIN (one-shot grid) or BFLY_VARIANT_STREAM.
+// variant: BFLY_VARIANT_PLAIN (one-shot grid), BFLY_VARIANT_STREAM or
+// BFLY_VARIANT_STREAM_LT; `adjoint` runs the transform's adjoint (R2C: the
+// C2R's adjoint weighting of the bins; C2R: the R2C's).
 hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                uint32_t stride, int variant, const float* table, hipStream_t s);
+                uint32_t stride, int variant, bool adjoint, const float* table, hipStream_t s);
 hipError_t copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// Per-mode channel mixing of the spectral convolution (bfly_mix.hip);
+// op 0 forward, 1 input gradient, 2 weight gradient (see bfly_mode_mix).
+hipError_t mode_mix(int op, const float* a, const float* b, float* c, uint32_t batch, uint32_t c_in,
+                    uint32_t c_out, uint32_t modes, hipStream_t s);
 // Split-FP16 matrix-core C2C (bfly_mfma.hip), n = 1024 / 2048 / 4096;
 // `stream` selects the resident-grid form.
 hipError_t c2c_mfma(const float* in, float* out, uint32_t n, uint32_t batch, int sign, bool stream,

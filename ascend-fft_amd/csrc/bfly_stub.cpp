@@ -110,8 +110,9 @@ int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const floa
     return launch();
 }
 int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                 uint32_t stride, int variant, const float* table, void*) {
+                 uint32_t stride, int variant, uint32_t flags, const float* table, void*) {
     if (kind != BFLY_KIND_R2C && kind != BFLY_KIND_C2R) return BFLY_ERR_SHAPE;
+    if (flags & ~BFLY_REAL_ADJOINT) return BFLY_ERR_SHAPE;
     if (int e = bfly_host::check_execute(kind, kind == BFLY_KIND_R2C ? -1 : 1, in, out, n, batch, table,
                                          nullptr))
         return e;
@@ -146,6 +147,12 @@ int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint
     }
     *ms = 0.001f * (float)reps;
     return BFLY_OK;
+}
+
+int bfly_mode_mix(int op, const float* a, const float* b, float* c, uint32_t batch, uint32_t c_in,
+                  uint32_t c_out, uint32_t modes, void*) {
+    if (int e = bfly_host::check_mode_mix(op, a, b, c, batch, c_in, c_out, modes)) return e;
+    return launch();
 }
 
 // --------------------------------------------------------------- probes

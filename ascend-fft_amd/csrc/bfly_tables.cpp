@@ -189,6 +189,35 @@ bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride) {
     return bins >= 1 && bins <= n / 2 + 1 && stride >= 2 * bins && (stride % 2) == 0;
 }
 
+int check_mode_mix(int op, const float* a, const float* b, const float* c, uint32_t batch, uint32_t c_in,
+                   uint32_t c_out, uint32_t modes) {
+    if (op < BFLY_MIX_FORWARD || op > BFLY_MIX_GRAD_WEIGHT) return BFLY_ERR_SHAPE;
+    if (!a || !b || !c || batch == 0 || c_in == 0 || c_out == 0 || modes == 0) return BFLY_ERR_SHAPE;
+    for (const float* p : {a, b, c})
+        if ((uintptr_t)p % 8) return BFLY_ERR_SHAPE;
+    // element counts, refused when a product leaves 64 bits (no device holds them)
+    uint64_t x, w, y;
+    if (__builtin_mul_overflow((uint64_t)batch * c_in, (uint64_t)modes, &x) ||
+        __builtin_mul_overflow((uint64_t)c_in * c_out, (uint64_t)modes, &w) ||
+        __builtin_mul_overflow((uint64_t)batch * c_out, (uint64_t)modes, &y) || x >> 60 || w >> 60 || y >> 60)
+        return BFLY_ERR_SHAPE;
+    // complex elements of a, b, c and the (m, n) extents of c for each op
+    const uint64_t na = op == BFLY_MIX_GRAD_INPUT ? y : x;
+    const uint64_t nb = op == BFLY_MIX_GRAD_WEIGHT ? y : w;
+    const uint64_t nc = op == BFLY_MIX_FORWARD ? y : op == BFLY_MIX_GRAD_INPUT ? x : w;
+    const uint64_t m = op == BFLY_MIX_GRAD_WEIGHT ? c_in : batch;
+    const uint64_t n = op == BFLY_MIX_GRAD_INPUT ? c_in : c_out;
+    // tiles of 16 modes x 16 x 16; each factor < 2^29, so no product overflows
+    const uint64_t tiles2 = ((uint64_t)modes + 15) / 16 * ((m + 15) / 16);
+    if (tiles2 > 0x7FFFFFFFull || tiles2 * ((n + 15) / 16) > 0x7FFFFFFFull) return BFLY_ERR_SHAPE;
+    auto overlap = [](const float* p, uint64_t np, const float* q, uint64_t nq) {
+        const uintptr_t p0 = (uintptr_t)p, q0 = (uintptr_t)q;
+        return p0 < q0 + 8 * nq && q0 < p0 + 8 * np;
+    };
+    if (overlap(c, nc, a, na) || overlap(c, nc, b, nb)) return BFLY_ERR_SHAPE;
+    return BFLY_OK;
+}
+
 bool group_ok(int kind, uint32_t n, int wg) {
     if (wg != 512 && wg != 1024) return false;
     if (!pow2(n)) return false;

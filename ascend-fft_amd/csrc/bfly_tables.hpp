@@ -52,6 +52,12 @@ int auto_variant(uint32_t n, uint32_t batch);
 // The variant BFLY_VARIANT_AUTO resolves to for a real transform of (n, batch).
 int auto_real_variant(uint32_t n, uint32_t batch);
 
+// Guards of bfly_mode_mix: op 0..2, non-null 8-byte aligned operands, every
+// size >= 1, an output overlapping neither input, a grid that fits.
+// BFLY_OK or BFLY_ERR_SHAPE.
+int check_mode_mix(int op, const float* a, const float* b, const float* c, uint32_t batch, uint32_t c_in,
+                   uint32_t c_out, uint32_t modes);
+
 // ------------------------------------------------------- grouped launch
 // Whether a grouped launch with `wg` threads per workgroup takes (kind, n).
 bool group_ok(int kind, uint32_t n, int wg);

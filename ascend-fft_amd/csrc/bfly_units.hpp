@@ -207,12 +207,16 @@ __device__ __forceinline__ void r2c_load(float2 (&v)[bfly_plan::kPoints], const 
 }
 
 // Compute phase: FFT_M, spectrum parked in LDS, split post-twiddle, store.
+// `adj` (uniform) stores the adjoint of the C2R instead: X[k] v_k / n with
+// v = 1 at bins 0 and M (imaginary part zero; the C2R reads only the real
+// parts there) and 2 elsewhere.  Every factor is a power of two, so this is
+// bit for bit the forward spectrum rescaled afterwards.
 // Ends with LDS reads still in flight in other threads: a caller that reuses
 // `lds` synchronises first.
 template <int I, int WG, int SC = 0, class TWS = RealTwGlobal<I, SC>>
 __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], float* __restrict__ out,
                                            const TWS& tws, uint32_t batch, uint32_t kout, uint32_t ostride,
-                                           uint32_t g, float2* lds, int tid) {
+                                           uint32_t g, float2* lds, int tid, bool adj = false) {
     using PL = PlanOf<I, SC>;
     constexpr int M = PL::N, T = PL::T, B = WG / T, RL = PL::RL, P = PL::P;
     static_assert(WG % T == 0, "a work unit holds whole transforms");
@@ -240,7 +244,13 @@ __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], floa
                 const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
                 const float2 w = tws.w(k);
                 const float2 wo = cmul_tw<-1>(o, w);
-                st_io<(T >= 16)>(&dst[k], cadd(e, wo));
+                float2 x = cadd(e, wo);
+                if (adj) {
+                    const bool edge = k == 0 || k == M;
+                    const float f = edge ? 0.5f / (float)M : 1.0f / (float)M;
+                    x = make_float2(x.x * f, edge ? 0.f : x.y * f);
+                }
+                st_io<(T >= 16)>(&dst[k], x);
             }
         }
     }
@@ -281,12 +291,15 @@ __device__ __forceinline__ void c2r_load(float2 (&q)[bfly_plan::kPoints + 1], co
 }
 
 // Compute phase: bins staged in LDS, Hermitian pre-twiddle, inverse FFT_M,
-// reals scaled and stored.  Starts by writing `lds`: a caller that reuses it
-// synchronises first.
+// reals scaled and stored.  `adj` (uniform) doubles the k = 0 term of the
+// pre-twiddle (bins 0 and M); with scale = 1/2 the result is the adjoint of
+// the R2C, n * C2R(w X) with w = 1 at bins 0 and M and 1/2 elsewhere, bit for
+// bit (every factor is a power of two).  Starts by writing `lds`: a caller
+// that reuses it synchronises first.
 template <int I, int WG, int SC = 0, class TWS = RealTwGlobal<I, SC>>
 __device__ __forceinline__ void c2r_finish(float2 (&q)[bfly_plan::kPoints + 1], float* __restrict__ out,
                                            const TWS& tws, uint32_t batch, float scale, uint32_t g,
-                                           float2* lds, int tid) {
+                                           float2* lds, int tid, bool adj = false) {
     using PL = PlanOf<I, SC>;
     constexpr int M = PL::N, T = PL::T, B = WG / T, R0 = PL::R0, RL = PL::RL, P = PL::P;
     static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
@@ -307,7 +320,11 @@ __device__ __forceinline__ void c2r_finish(float2 (&q)[bfly_plan::kPoints + 1], 
         for (int r = 0; r < R0; r++) {
             const int k = t + m * T + r * (M / R0);
             float2 a = L[k], b = L[M - k];
-            if (k == 0) { a.y = 0.f; b.y = 0.f; }
+            if (k == 0) {
+                a.y = 0.f;
+                b.y = 0.f;
+                if (adj) { a.x *= 2.f; b.x *= 2.f; }
+            }
             const float2 s = make_float2(a.x + b.x, a.y - b.y);
             const float2 d = make_float2(a.x - b.x, a.y + b.y);
             const float2 wd = cmul_tw<1>(d, tws.w(k));  // W_n^{-k} d

@@ -162,7 +162,7 @@ def secondary_rows(cpu_seconds: float, reps: int = 20):
     for kind, name, src, dst in ((1, "r2c_n4096_b4096", xr, hs), (2, "c2r_n4096_b4096", hs, xr)):
         tab = ops.table(kind, n, dev)
         fn = (lambda k, t, a, c: lambda i: _lib.check(lib.bfly_real_ex(
-            k, a[i % 4].data_ptr(), c[i % 4].data_ptr(), n, b, n // 2 + 1, n + 2, _lib.VARIANT_AUTO,
+            k, a[i % 4].data_ptr(), c[i % 4].data_ptr(), n, b, n // 2 + 1, n + 2, _lib.VARIANT_AUTO, 0,
             t.data_ptr(), sid()), "real"))(kind, tab, src, dst)
         out[name] = row(timed(fn), (4 * n + 4 * (n + 2)) * b, b)
     del xr, hs
@@ -359,7 +359,7 @@ def workload_loop(name: str, steps: int, warmup: int):
         tab = ops.table(kind, n, dev)
         src, dst = (x, torch.empty_like(h)) if kind == 1 else (h, torch.empty_like(x))
         fn = lambda: _lib.check(lib.bfly_real_ex(kind, src.data_ptr(), dst.data_ptr(), n, b, n // 2 + 1,  # noqa: E731
-                                                 n + 2, _lib.VARIANT_AUTO, tab.data_ptr(), sid()), name)
+                                                 n + 2, _lib.VARIANT_AUTO, 0, tab.data_ptr(), sid()), name)
         nbytes = (4 * n + 4 * (n + 2)) * b
     elif name in ("mfma", "long65536", "single16384"):
         n, b, v = {"mfma": (4096, 4096, _lib.VARIANT_SPLIT_MFMA), "long65536": (65536, 256, _lib.VARIANT_AUTO),

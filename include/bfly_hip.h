@@ -181,10 +181,22 @@ BFLY_API int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch
  * BFLY_VARIANT_STREAM_LT = streaming with the twiddles in LDS (two-level
  * products; n >= 256, else STREAM), BFLY_VARIANT_AUTO = the library's
  * choice; other ids give BFLY_ERR_SHAPE.  Same reference interfaces as
- * bfly_r2c / bfly_c2r. */
+ * bfly_r2c / bfly_c2r.
+ *
+ * flags: 0, or BFLY_REAL_ADJOINT for the transform's adjoint in the same
+ * layouts (the backward pass of a differentiable FFT, one launch, no extra
+ * weighting pass):
+ *   R2C + ADJOINT: out[k] = X[k] * v_k / n, v_k = 1 at bins 0 and n/2 (with
+ *     the imaginary part set to 0) and 2 elsewhere -- the adjoint of C2R;
+ *   C2R + ADJOINT: out = n * C2R(w X), w_k = 1 at bins 0 and n/2 and 1/2
+ *     elsewhere -- the adjoint of R2C.
+ * Every weight is a power of two: the results equal the forward transforms
+ * followed by the weighting, bit for bit.  Other flag bits give
+ * BFLY_ERR_SHAPE.  The reference has no adjoint transform. */
+#define BFLY_REAL_ADJOINT 1u
 BFLY_API int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch,
-                          uint32_t bins, uint32_t stride, int variant, const float* table,
-                          void* stream);
+                          uint32_t bins, uint32_t stride, int variant, uint32_t flags,
+                          const float* table, void* stream);
 
 /* ------------------------------------------------------- grouped launch
  * Many independent transforms of different kinds, lengths and batch sizes
@@ -244,6 +256,27 @@ BFLY_API int bfly_group_launch(void* group, unsigned long long* stamps, void* st
 /* units = work units, grid = workgroups launched, bytes = device bytes held. */
 BFLY_API int bfly_group_query(void* group, uint32_t* units, uint32_t* grid, size_t* bytes);
 BFLY_API int bfly_group_destroy(void* group);
+
+/* ------------------------------------------------- spectral convolution
+ * Per-mode channel mixing of the 1-D spectral-convolution layer, in the
+ * layouts bfly_r2c_ex / bfly_c2r_ex use for truncated modes (complex64
+ * interleaved, mode index fastest):
+ *   x  [batch][c_in][modes]   input modes        (bfly_r2c_ex, bins = modes)
+ *   w  [c_in][c_out][modes]   weights
+ *   y  [batch][c_out][modes]  output modes       (bfly_c2r_ex, bins = modes)
+ * op BFLY_MIX_FORWARD:      a = x,  b = w,  c = y,   y[b][o][k] = sum_i x[b][i][k] w[i][o][k]
+ * op BFLY_MIX_GRAD_INPUT:   a = gy, b = w,  c = gx,  gx[b][i][k] = sum_o gy[b][o][k] conj(w[i][o][k])
+ * op BFLY_MIX_GRAD_WEIGHT:  a = x,  b = gy, c = gw,  gw[i][o][k] = sum_b conj(x[b][i][k]) gy[b][o][k]
+ * (the gradients follow torch's convention for complex tensors).  Every
+ * element of c is written; c must not overlap a or b.  fp32 multiply-adds,
+ * fp32 accumulation.  No counterpart in the reference (which has no
+ * spectral layer); it replaces the transposed copies plus batched GEMM a
+ * torch implementation of the mixing needs. */
+#define BFLY_MIX_FORWARD 0
+#define BFLY_MIX_GRAD_INPUT 1
+#define BFLY_MIX_GRAD_WEIGHT 2
+BFLY_API int bfly_mode_mix(int op, const float* a, const float* b, float* c, uint32_t batch, uint32_t c_in,
+                           uint32_t c_out, uint32_t modes, void* stream);
 
 /* ------------------------------------------------------------- probes */
 /* Device-to-device copy of `bytes` (16-byte aligned, multiple of 16) with

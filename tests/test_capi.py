@@ -158,9 +158,22 @@ def test_execute_guards_through_stub():
     assert s.bfly_r2c(p, p, 16, 1, p, None) == bfly.ERR_SHAPE
     assert s.bfly_c2r(p, p, 16384, 1, p, None) == bfly.ERR_SHAPE
     assert s.bfly_c2c(p, p, 4096, 0x7FFFFFFF, -1, p, None, None) == 0  # grid fits
+    assert s.bfly_real_ex(1, p, p, 1024, 1, 513, 1026, 0, _lib.REAL_ADJOINT, p, None) == 0
+    assert s.bfly_real_ex(2, p, p, 1024, 1, 40, 80, _lib.VARIANT_STREAM_LT, _lib.REAL_ADJOINT, p, None) == 0
+    assert s.bfly_real_ex(1, p, p, 1024, 1, 513, 1026, 0, 2, p, None) == bfly.ERR_SHAPE  # unknown flag
+    assert s.bfly_real_ex(2, p, p, 1024, 1, 513, 1026, _lib.VARIANT_STREAM2, 0, p, None) == bfly.ERR_SHAPE
+    # mode mixing: x [2,3,5], w [3,4,5], y [2,4,5] complex in one buffer, apart
+    x, w, y = p, p + 8 * 30, p + 8 * (30 + 60)
+    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y, 2, 3, 4, 5, None) == 0
+    assert s.bfly_mode_mix(_lib.MIX_GRAD_WEIGHT, x, y, w, 2, 3, 4, 5, None) == 0
+    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y - 8, 2, 3, 4, 5, None) == bfly.ERR_SHAPE  # c overlaps b
+    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y + 4, 2, 3, 4, 5, None) == bfly.ERR_SHAPE  # misaligned
+    assert s.bfly_mode_mix(3, x, w, y, 2, 3, 4, 5, None) == bfly.ERR_SHAPE
+    assert s.bfly_mode_mix(_lib.MIX_GRAD_INPUT, x, w, y, 2, 3, 4, 0, None) == bfly.ERR_SHAPE
+    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y, 1 << 31, 1 << 31, 1 << 31, 1 << 31, None) == bfly.ERR_SHAPE
     cnt = (C.c_int * 8)()
     s.bfly_stub_counters(cnt)
-    assert cnt[3] == 5  # launches counted only for accepted calls
+    assert cnt[3] == 9  # launches counted only for accepted calls
 
 
 def test_python_modules_import_and_variant_ids_match_header():
@@ -176,3 +189,8 @@ def test_python_modules_import_and_variant_ids_match_header():
                  "ERR_SYNC", "ERR_D2H", "ERR_SHAPE"):
         m = re.search(rf"#define BFLY_{name} \(?(-?\d+)\)?", text)
         assert m and int(m.group(1)) == getattr(_lib, name), name
+    m = re.search(r"#define BFLY_REAL_ADJOINT (\d+)u", text)
+    assert m and int(m.group(1)) == _lib.REAL_ADJOINT
+    for name in ("FORWARD", "GRAD_INPUT", "GRAD_WEIGHT"):
+        m = re.search(rf"#define BFLY_MIX_{name} (\d+)", text)
+        assert m and int(m.group(1)) == getattr(_lib, f"MIX_{name}"), name

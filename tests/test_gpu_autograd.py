@@ -78,6 +78,121 @@ def test_mode_truncated_real_transforms(n, modes):
         assert oracle.rel_l2_rows(back, oracle.c2r_f64(full, n)) <= 1e-5
 
 
+def _real_ex(kind, src, dst, n, b, bins, stride, variant, flags):
+    import torch
+    from bfly import _lib, ops
+    rc = _lib.hip().bfly_real_ex(kind, src.data_ptr(), dst.data_ptr(), n, b, bins, stride, variant, flags,
+                                 ops.table(kind, n, src.device).data_ptr(),
+                                 torch.cuda.current_stream().cuda_stream)
+    assert rc == 0, (kind, n, b, bins, variant, flags, rc)
+
+
+@pytest.mark.parametrize("n", [32, 256, 1024, 4096, 8192])
+def test_adjoint_real_kernels_equal_weighted_forward_bitwise(n):
+    # bfly_real_ex with BFLY_REAL_ADJOINT in every kernel form equals the
+    # forward launch of the same form with the adjoint weights applied
+    # outside (all powers of two, so exact), bit for bit, full and truncated.
+    import torch
+    import oracle
+    from bfly import _lib
+    M = n // 2
+    for b, bins in ((3, M + 1), (257, M + 1), (130, max(1, M // 8))):
+        x = torch.from_numpy(oracle.uniform_input(40 + b, (b, n))).cuda()
+        h = torch.from_numpy(oracle.uniform_input(50 + b, (b, 2 * bins))).cuda()
+        v = torch.full((bins,), 1.0 / M, device="cuda")
+        v[0] = 0.5 / M
+        if bins == M + 1:
+            v[M] = 0.5 / M
+        w = torch.full((bins,), 0.5, device="cuda")
+        w[0] = 1.0
+        if bins == M + 1:
+            w[M] = 1.0
+        for variant in (_lib.VARIANT_PLAIN, _lib.VARIANT_STREAM, _lib.VARIANT_STREAM_LT, _lib.VARIANT_AUTO):
+            fwd = torch.full((b, 2 * bins), float("nan"), device="cuda")
+            adj = torch.full((b, 2 * bins), float("nan"), device="cuda")
+            _real_ex(1, x, fwd, n, b, bins, 2 * bins, variant, 0)
+            _real_ex(1, x, adj, n, b, bins, 2 * bins, variant, _lib.REAL_ADJOINT)
+            want = (fwd.view(b, bins, 2) * v[:, None]).reshape(b, 2 * bins)
+            want[:, 1] = 0.0
+            if bins == M + 1:
+                want[:, 2 * M + 1] = 0.0
+            torch.cuda.synchronize()
+            assert torch.equal(adj, want), ("r2c", n, b, bins, variant)
+            hw = (h.view(b, bins, 2) * w[:, None]).reshape(b, 2 * bins).contiguous()
+            back = torch.full((b, n), float("nan"), device="cuda")
+            badj = torch.full((b, n), float("nan"), device="cuda")
+            _real_ex(2, hw, back, n, b, bins, 2 * bins, variant, 0)
+            _real_ex(2, h, badj, n, b, bins, 2 * bins, variant, _lib.REAL_ADJOINT)
+            torch.cuda.synchronize()
+            assert torch.equal(badj, back * n), ("c2r", n, b, bins, variant)
+
+
+@pytest.mark.parametrize("n,bins", [(64, 33), (1024, 513), (1024, 40), (8192, 4097)])
+def test_adjoint_real_kernels_satisfy_the_adjoint_identity_fp64(n, bins):
+    # <C2R h, g> = <h, R2C_adj g> and <R2C x, k> = <x, C2R_adj k> with the
+    # left sides from the FP64 oracle (real inner products; complex bins as
+    # pairs of reals).  A wrong weight, a missed edge or a wrong scale breaks
+    # them.
+    import torch
+    import oracle
+    from bfly import _lib
+    b = 6
+    M = n // 2
+    h = oracle.uniform_input(60, (b, 2 * bins)).astype(np.float64)
+    h[:, 1] = 0.0  # the C2R reads only the real part of bin 0 (and of bin M)
+    if bins == M + 1:
+        h[:, 2 * M + 1] = 0.0
+    g = oracle.uniform_input(61, (b, n)).astype(np.float64)
+    full = np.zeros((b, n + 2), np.float64)
+    full[:, :2 * bins] = h
+    lhs = np.sum(oracle.c2r_f64(full.astype(np.float32), n).astype(np.float64) * g, axis=1)
+    ga = torch.from_numpy(g.astype(np.float32)).cuda()
+    adj = torch.empty(b, 2 * bins, device="cuda")
+    _real_ex(1, ga, adj, n, b, bins, 2 * bins, _lib.VARIANT_AUTO, _lib.REAL_ADJOINT)
+    rhs = np.sum(h * adj.cpu().numpy().astype(np.float64), axis=1)
+    scale = np.sqrt(np.sum(lhs ** 2))
+    assert np.max(np.abs(lhs - rhs)) <= 1e-5 * scale, (lhs, rhs)
+
+    x = oracle.uniform_input(62, (b, n)).astype(np.float64)
+    k = oracle.uniform_input(63, (b, 2 * bins)).astype(np.float64)
+    spec = oracle.r2c_f64(x.astype(np.float32), n).astype(np.float64)[:, :2 * bins]
+    lhs = np.sum(spec * k, axis=1)
+    ka = torch.from_numpy(k.astype(np.float32)).cuda()
+    cadj = torch.empty(b, n, device="cuda")
+    _real_ex(2, ka, cadj, n, b, bins, 2 * bins, _lib.VARIANT_AUTO, _lib.REAL_ADJOINT)
+    rhs = np.sum(x * cadj.cpu().numpy().astype(np.float64), axis=1)
+    scale = np.sqrt(np.sum(lhs ** 2))
+    assert np.max(np.abs(lhs - rhs)) <= 1e-5 * scale, (lhs, rhs)
+
+
+@pytest.mark.parametrize("B,C_in,C_out,K", [(2, 4, 4, 32), (3, 5, 7, 19), (17, 40, 24, 37), (64, 128, 128, 256)])
+def test_mode_mix_kernel_matches_fp64(B, C_in, C_out, K):
+    # bfly_mode_mix forward / input gradient / weight gradient against
+    # complex128 einsums of the same operands (ragged tiles at every edge)
+    import torch
+    from bfly import _lib, ops
+    g = torch.Generator().manual_seed(B * 1000 + K)
+
+    def crand(*shape):
+        return torch.complex(torch.randn(*shape, generator=g), torch.randn(*shape, generator=g))
+    x, w, gy = crand(B, C_in, K), crand(C_in, C_out, K), crand(B, C_out, K)
+    x64, w64, gy64 = x.to(torch.complex128), w.to(torch.complex128), gy.to(torch.complex128)
+    want = {_lib.MIX_FORWARD: torch.einsum("bik,iok->bok", x64, w64),
+            _lib.MIX_GRAD_INPUT: torch.einsum("bok,iok->bik", gy64, w64.conj()),
+            _lib.MIX_GRAD_WEIGHT: torch.einsum("bik,bok->iok", x64.conj(), gy64)}
+    args = {_lib.MIX_FORWARD: (x, w), _lib.MIX_GRAD_INPUT: (gy, w), _lib.MIX_GRAD_WEIGHT: (x, gy)}
+    for op, (a, b) in args.items():
+        out = ops.mode_mix(op, a.cuda(), b.cuda(), B, C_in, C_out, K)
+        torch.cuda.synchronize()
+        assert _rel(out.cpu().numpy(), want[op].numpy()) < 1e-6, op
+    # the output must not overlap an input; unknown ops are refused
+    a, b = x.cuda(), w.cuda()
+    assert _lib.hip().bfly_mode_mix(0, a.data_ptr(), b.data_ptr(), a.data_ptr(), B, C_in, C_out, K,
+                                    None) == _lib.ERR_SHAPE
+    assert _lib.hip().bfly_mode_mix(3, a.data_ptr(), b.data_ptr(), a.data_ptr(), B, C_in, C_out, K,
+                                    None) == _lib.ERR_SHAPE
+
+
 def _fp64_layer(x, w):
     import torch
     K = w.shape[-1]

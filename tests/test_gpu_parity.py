@@ -155,7 +155,7 @@ def test_real_streaming_variant_equals_one_shot(n):
         outs = []
         for v in (_lib.VARIANT_PLAIN, _lib.VARIANT_STREAM):
             y = torch.full((b, 2 * bins), float("nan"), device="cuda")
-            assert lib.bfly_real_ex(1, xt.data_ptr(), y.data_ptr(), n, b, bins, 2 * bins, v,
+            assert lib.bfly_real_ex(1, xt.data_ptr(), y.data_ptr(), n, b, bins, 2 * bins, v, 0,
                                     tab.data_ptr(), s) == 0
             outs.append(y)
         torch.cuda.synchronize()
@@ -171,7 +171,7 @@ def test_real_streaming_variant_equals_one_shot(n):
         backs = []
         for v in (_lib.VARIANT_PLAIN, _lib.VARIANT_STREAM):
             z = torch.full((b, n), float("nan"), device="cuda")
-            assert lib.bfly_real_ex(2, h.data_ptr(), z.data_ptr(), n, b, bins, n + 2, v,
+            assert lib.bfly_real_ex(2, h.data_ptr(), z.data_ptr(), n, b, bins, n + 2, v, 0,
                                     tab2.data_ptr(), s) == 0
             backs.append(z)
         torch.cuda.synchronize()
@@ -179,7 +179,7 @@ def test_real_streaming_variant_equals_one_shot(n):
         assert oracle.rel_l2_rows(backs[1].cpu().numpy(), oracle.c2r_f64(h.cpu().numpy(), n)) <= REL_L2
     # ids other than AUTO / PLAIN / STREAM / STREAM_LT are refused
     assert lib.bfly_real_ex(1, xt.data_ptr(), xt.data_ptr(), n, 1, n // 2 + 1, n + 2,
-                            _lib.VARIANT_STREAM2, tab.data_ptr(), s) == _lib.ERR_SHAPE
+                            _lib.VARIANT_STREAM2, 0, tab.data_ptr(), s) == _lib.ERR_SHAPE
 
 
 @pytest.mark.parametrize("n", [256, 1024, 4096, 8192])
@@ -194,7 +194,7 @@ def test_real_lds_twiddle_variant_fp64(n):
         x = oracle.uniform_input(33, (b, n))
         xt = torch.from_numpy(x).cuda()
         y = torch.full((b, 2 * bins), float("nan"), device="cuda")
-        assert lib.bfly_real_ex(1, xt.data_ptr(), y.data_ptr(), n, b, bins, 2 * bins, _lib.VARIANT_STREAM_LT,
+        assert lib.bfly_real_ex(1, xt.data_ptr(), y.data_ptr(), n, b, bins, 2 * bins, _lib.VARIANT_STREAM_LT, 0,
                                 _ops().table(1, n, xt.device).data_ptr(), s) == 0
         torch.cuda.synchronize()
         ref = oracle.r2c_f64(x, n)[:, :2 * bins]
@@ -205,7 +205,7 @@ def test_real_lds_twiddle_variant_fp64(n):
         if bins == n // 2 + 1:
             h[:, n + 1] = 0
         z = torch.full((b, n), float("nan"), device="cuda")
-        assert lib.bfly_real_ex(2, h.data_ptr(), z.data_ptr(), n, b, bins, n + 2, _lib.VARIANT_STREAM_LT,
+        assert lib.bfly_real_ex(2, h.data_ptr(), z.data_ptr(), n, b, bins, n + 2, _lib.VARIANT_STREAM_LT, 0,
                                 _ops().table(2, n, xt.device).data_ptr(), s) == 0
         torch.cuda.synchronize()
         assert oracle.rel_l2_rows(z.cpu().numpy(), oracle.c2r_f64(h.cpu().numpy(), n)) <= REL_L2
