@@ -75,7 +75,7 @@ class GroupRequest(C.Structure):
 class GroupConfig(C.Structure):
     """bfly_group_config of include/bfly_hip.h."""
     _fields_ = [("workgroup", C.c_int32), ("schedule", C.c_int32), ("order", C.c_int32),
-                ("dynamic", C.c_int32), ("grid", C.c_int32)]
+                ("dynamic", C.c_int32), ("grid", C.c_int32), ("twiddles", C.c_int32)]
 
 
 class BflyError(RuntimeError):

@@ -11,8 +11,9 @@ the kind's execute layout:
 The configuration axes the tuner searches are the ones the kernel exposes:
 threads per workgroup (512 / 1024, which fixes the transforms per workgroup
 at 512*16/n' or 1024*16/n'), radix schedule (radix 16 first or the
-remainder radix first), unit order, static or atomic-counter scheduling and
-the number of persistent workgroups.
+remainder radix first), unit order, static or atomic-counter scheduling,
+the number of persistent workgroups and where the inter-pass twiddles come
+from (device memory, or the two-level tables staged in LDS per unit).
 """
 from __future__ import annotations
 
@@ -35,13 +36,23 @@ class Config:
     order: int = 0
     dynamic: int = 0
     grid: int = 0
+    twiddles: int = 0
 
     def as_c(self) -> _lib.GroupConfig:
-        return _lib.GroupConfig(self.workgroup, self.schedule, self.order, self.dynamic, self.grid)
+        return _lib.GroupConfig(self.workgroup, self.schedule, self.order, self.dynamic, self.grid,
+                                self.twiddles)
 
     def name(self) -> str:
         return (f"wg{self.workgroup}_sched{self.schedule}_order{self.order}_"
-                f"{'dyn' if self.dynamic else 'static'}_grid{self.grid or 'res'}")
+                f"{'dyn' if self.dynamic else 'static'}_grid{self.grid or 'res'}_tw{self.twiddles}")
+
+    def valid(self) -> bool:
+        """LDS twiddle tables run radix schedule 0 only."""
+        return not (self.twiddles == 1 and self.schedule != 0)
+
+    def reference_variant(self) -> int:
+        """The per-request variant whose kernels compute the same bits."""
+        return _lib.VARIANT_STREAM_LT if self.twiddles == 1 else _lib.VARIANT_PLAIN
 
 
 @dataclass
@@ -148,8 +159,8 @@ class Group:
 
 
 def launch_per_request(reqs: list[Request], variant: int = _lib.VARIANT_PLAIN) -> None:
-    """The same requests as one library call each (bfly_c2c_ex / bfly_r2c /
-    bfly_c2r) on the current stream."""
+    """The same requests as one library call each (bfly_c2c_ex / bfly_real_ex
+    of the given variant) on the current stream."""
     import torch
     from . import ops
     lib = _lib.hip()
@@ -160,10 +171,9 @@ def launch_per_request(reqs: list[Request], variant: int = _lib.VARIANT_PLAIN) -
             work = ops.workspace(r.kind, r.n, r.batch, r.x.device)
             rc = lib.bfly_c2c_ex(r.x.data_ptr(), r.y.data_ptr(), r.n, r.batch, r.sign, variant,
                                  tab.data_ptr(), work.data_ptr() if work is not None else None, s)
-        elif r.kind == _lib.KIND_R2C:
-            rc = lib.bfly_r2c(r.x.data_ptr(), r.y.data_ptr(), r.n, r.batch, tab.data_ptr(), s)
         else:
-            rc = lib.bfly_c2r(r.x.data_ptr(), r.y.data_ptr(), r.n, r.batch, tab.data_ptr(), s)
+            rc = lib.bfly_real_ex(r.kind, r.x.data_ptr(), r.y.data_ptr(), r.n, r.batch, r.n // 2 + 1, r.n + 2,
+                                  variant, 0, tab.data_ptr(), s)
         _lib.check(rc, f"per-request {KIND_NAMES[r.kind]} n={r.n} batch={r.batch}")
 
 
@@ -254,6 +264,7 @@ DEFAULT_SPACE = {
     "order": (0, 1, 2, 3),
     "dynamic": (0, 1),
     "grid": (0,),
+    "twiddles": (0, 1),
 }
 
 
@@ -281,7 +292,8 @@ def autotune(reqs: list[Request], space: dict | None = None, reps: int = 10,
     keys = list(space)
     for vals in itertools.product(*(space[k] for k in keys)):
         cfg = Config(**dict(zip(keys, vals)))
-        if any(not _lib.hip().bfly_group_supported(r.kind, r.n, cfg.workgroup) for r in reqs):
+        if not cfg.valid() or any(not _lib.hip().bfly_group_supported(r.kind, r.n, cfg.workgroup)
+                                  for r in reqs):
             continue
         grp = Group(reqs, cfg)
         for r in reqs:

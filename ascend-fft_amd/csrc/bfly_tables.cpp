@@ -234,11 +234,12 @@ bool group_ok(int kind, uint32_t n, int wg) {
 
 int group_plan(const bfly_group_request* reqs, int nreq, const bfly_group_config* cfg, GroupPlan* out) {
     if (!reqs || nreq <= 0 || !out) return BFLY_ERR_SHAPE;
-    bfly_group_config c{512, 0, 0, 0, 0};
+    bfly_group_config c{512, 0, 0, 0, 0, 0};
     if (cfg) c = *cfg;
     if (c.workgroup == 0) c.workgroup = 512;
     if ((c.workgroup != 512 && c.workgroup != 1024) || (c.schedule != 0 && c.schedule != 1) ||
-        c.order < 0 || c.order > 3 || (c.dynamic != 0 && c.dynamic != 1) || c.grid < 0)
+        c.order < 0 || c.order > 3 || (c.dynamic != 0 && c.dynamic != 1) || c.grid < 0 ||
+        (c.twiddles != 0 && c.twiddles != 1) || (c.twiddles == 1 && c.schedule != 0))
         return BFLY_ERR_SHAPE;
     GroupPlan& P = *out;
     P = GroupPlan{};

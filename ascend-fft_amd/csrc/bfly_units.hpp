@@ -106,6 +106,64 @@ __device__ __forceinline__ void c2c_unit(const float2* __restrict__ in, float2* 
     }
 }
 
+// The same unit with the inter-pass twiddles read from LDS tables (the
+// per-transform code of k_c2c_stream_lt): `ltw` = [A (NLO) | B (NHI) | the
+// second pass's table stored [r 16 + k]], filled by c2c_lds_tw.
+template <int I>
+struct C2cLdsTw {
+    static constexpr int N = kPlans[I].n;
+    static constexpr int NLO = N < TW2_LO ? N : TW2_LO;
+    static constexpr int NHI = N / NLO;
+    static constexpr int R1 = kPlans[I].rad[kPlans[I].nrad > 1 ? 1 : 0];
+    static constexpr int NT1 = kPlans[I].nrad > 1 ? bfly_plan::kPoints * R1 : 1;
+    static constexpr int LEN = NLO + NHI + NT1;
+};
+
+// Fills `buf` (C2cLdsTw<I>::LEN entries) from the plan's pass tables `tw` and
+// `ab` = [W_N^j, j < NLO | W_N^{NLO j}, j < NHI] in device memory.  The
+// caller synchronises before the tables are read.
+template <int I, int WG>
+__device__ __forceinline__ void c2c_lds_tw(float2* buf, const float2* __restrict__ tw,
+                                           const float2* __restrict__ ab, int tid) {
+    using L = C2cLdsTw<I>;
+    for (int i = tid; i < L::NLO + L::NHI; i += WG) buf[i] = ab[i];
+    if constexpr (kPlans[I].nrad > 1)
+        for (int i = tid; i < L::NT1; i += WG) buf[L::NLO + L::NHI + (i % L::R1) * bfly_plan::kPoints + i / L::R1] = tw[i];
+}
+
+template <int I, int DIR, int WG>
+__device__ __forceinline__ void c2c_unit_lt(const float2* __restrict__ in, float2* __restrict__ out,
+                                            const float2* ltw, uint32_t batch, uint32_t g, float2* lds, int tid) {
+    using PL = PlanOf<I>;
+    using L = C2cLdsTw<I>;
+    constexpr int N = PL::N, T = PL::T, B = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    const bool active = tr < batch;
+    float2 v[P];
+    if (active) {
+        const float2* src = in + tr * N;
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) v[m * R0 + r] = ld_io<(T >= 16)>(&src[t + m * T + r * (N / R0)]);
+    } else {
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = make_float2(0.f, 0.f);
+    }
+    run_plan_lt<I, T, DIR, LX_PAD>(v, lds + bl * lds_len(N), 0, ltw, ltw + L::NLO, ltw + L::NLO + L::NHI, t,
+                                   std::make_index_sequence<PL::NRAD>{});
+    if (active) {
+        float2* dst = out + tr * N;
+#pragma unroll
+        for (int m = 0; m < P / RL; m++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) st_io<(T >= 16)>(&dst[t + m * T + r * (N / RL)], v[m * RL + r]);
+    }
+}
+
 // --------------------------------------------------- real twiddle sources
 // Where a real unit takes its twiddles from: the plan's pass tables (for
 // the n/2-point FFT) and W_n^k, k <= n/2 (for the split twiddle).

@@ -201,9 +201,11 @@ def grouped_pattern(reps: int = 20):
     captured in a hipGraph, configuration chosen by the autotuner, against
     the same requests as 512 per-request launches (eager and captured).
     Outputs are checked against the per-request launches before timing."""
+    import dataclasses
+
     import numpy as np
     import torch
-    from bfly import group
+    from bfly import _lib, group
 
     specs = group.uneven_specs()
     reqs = group.make_requests(specs, seed=3)
@@ -219,13 +221,16 @@ def grouped_pattern(reps: int = 20):
     err = group.max_rel_diff(group.outputs(reqs), ref)
     graph = group.capture(g.launch)
     ms_group = group.time_ms(graph.replay, reps=reps)
-    per_graph = group.capture(lambda: group.launch_per_request(reqs))
+    # the per-request baseline: one library call per request, the library's
+    # own choice of kernel (AUTO)
+    auto = _lib.VARIANT_AUTO
+    per_graph = group.capture(lambda: group.launch_per_request(reqs, auto))
     ms_per_graph = group.time_ms(per_graph.replay, reps=reps)
-    ms_per_eager = group.time_ms(lambda: group.launch_per_request(reqs), reps=max(3, reps // 4))
+    ms_per_eager = group.time_ms(lambda: group.launch_per_request(reqs, auto), reps=max(3, reps // 4))
     lat = group.latencies_us(g)
-    lat_seq = group.per_request_latencies_us(reqs)
+    lat_seq = group.per_request_latencies_us(reqs, auto)
     # the same configuration with the least-work-first unit order (latency)
-    cfg_lat = group.Config(tuned.best.workgroup, tuned.best.schedule, 3, tuned.best.dynamic, tuned.best.grid)
+    cfg_lat = dataclasses.replace(tuned.best, order=3)
     g_lat = group.Group(reqs, cfg_lat)
     ms_lat = group.time_ms(g_lat.launch, reps=reps)
     lat_sjf = group.latencies_us(g_lat)
@@ -255,6 +260,7 @@ def grouped_pattern(reps: int = 20):
             "latency_us_p99": round(float(np.percentile(lat_sjf, 99)), 2),
         },
         "per_request_launches": {
+            "variant": "auto",
             "ms_graph": round(ms_per_graph, 4),
             "ms_eager": round(ms_per_eager, 4),
             "latency_us_p50": round(float(np.percentile(lat_seq, 50)), 2),

@@ -207,8 +207,9 @@ BFLY_API int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uin
  * uploaded once by bfly_group_create; bfly_group_launch only launches, so it
  * can be captured into a hipGraph and replayed (the table's device address
  * stays fixed until bfly_group_destroy).  Results are those of per-request
- * bfly_c2c_ex(PLAIN) / bfly_r2c / bfly_c2r calls: with schedule 0 the same
- * kernels' per-transform code runs, bit for bit.
+ * bfly_c2c_ex / bfly_real_ex calls of variant PLAIN (STREAM_LT with
+ * twiddles = 1): with schedule 0 the same kernels' per-transform code runs,
+ * bit for bit.
  *
  * Work is cut into units of 16*workgroup points (one HBM read and write
  * each): a unit is workgroup/(n'/16) transforms of one request, n' = n for
@@ -234,6 +235,12 @@ typedef struct bfly_group_config {
     int32_t dynamic;   /* 0 = workgroup w runs units w, w+grid, ...;
                           1 = workgroups take units from an atomic counter */
     int32_t grid;      /* workgroups launched; 0 = the resident capacity */
+    int32_t twiddles;  /* 0 = inter-pass twiddles read from device memory (the
+                          per-request PLAIN kernels' code); 1 = each unit stages
+                          its plan's two-level tables in LDS first (the
+                          per-request BFLY_VARIANT_STREAM_LT code, bit for bit;
+                          schedule 0 only; real kinds below n = 256 keep the
+                          device tables, as their STREAM_LT launches do) */
 } bfly_group_config;
 
 /* 1 when a grouped launch with `workgroup` threads (512 / 1024) takes

@@ -2,8 +2,8 @@
 
 With radix schedule 0 a grouped transform runs the same per-transform code
 as its per-request launch, so outputs must equal the per-request PLAIN
-launches bit for bit, in every launch configuration and under hipGraph
-replay.  Radix schedule 1 (remainder radix first) rounds differently and is
+launches (STREAM_LT launches with LDS twiddles) bit for bit, in every launch
+configuration and under hipGraph replay.  Radix schedule 1 (remainder radix first) rounds differently and is
 gated against the FP64 oracle (relative L2 <= 1e-5 per transform).
 """
 import itertools
@@ -81,6 +81,61 @@ def test_group_equals_per_request_launches_bitwise(mixed, wg, order, dynamic):
         torch.cuda.synchronize()
         for r, want in zip(reqs, ref):
             assert torch.equal(r.y, want), (r.kind, r.n, r.batch, r.sign)
+    g.close()
+
+
+@pytest.fixture(scope="module")
+def mixed_lt():
+    # the same requests' per-request STREAM_LT launches (C2R / R2C below
+    # n = 256 run their register-twiddle kernels, as the grouped LT path does)
+    import torch
+    from bfly import group
+    specs, signs = _mixed_specs()
+    reqs = group.make_requests(specs, seed=11, signs=signs)
+    group.launch_per_request(reqs, variant=group.Config(twiddles=1).reference_variant())
+    torch.cuda.synchronize()
+    return reqs, group.outputs(reqs)
+
+
+@pytest.mark.parametrize("wg,order,dynamic", list(itertools.product((512, 1024), (0, 3), (0, 1))))
+def test_group_lds_twiddles_equal_stream_lt_launches_bitwise(mixed_lt, wg, order, dynamic):
+    # twiddles = 1: each unit stages its plan's two-level tables in LDS and
+    # runs the STREAM_LT kernels' per-transform code -- the same bits
+    import torch
+    from bfly import group
+    reqs, ref = mixed_lt
+    g = group.Group(reqs, group.Config(workgroup=wg, order=order, dynamic=dynamic, twiddles=1))
+    for _ in range(2):
+        for r in reqs:
+            r.y.fill_(float("nan"))
+        g.launch()
+        torch.cuda.synchronize()
+        for r, want in zip(reqs, ref):
+            assert torch.equal(r.y, want), (r.kind, r.n, r.batch, r.sign)
+    _gate_fp64(reqs)
+    g.close()
+
+
+def test_group_lds_twiddles_16384_and_graph_replays():
+    import torch
+    from bfly import group
+    reqs = group.make_requests([(0, 16384, 3), (0, 8192, 2), (1, 8192, 3), (2, 4096, 5), (0, 16, 7)], seed=13,
+                               signs=[-1, 1, -1, 1, 1])
+    group.launch_per_request(reqs, variant=group.Config(twiddles=1).reference_variant())
+    torch.cuda.synchronize()
+    ref = group.outputs(reqs)
+    _gate_fp64(reqs)
+    g = group.Group(reqs, group.Config(workgroup=1024, twiddles=1, dynamic=1))
+    graph = group.capture(g.launch)
+    for rep in range(3):
+        for r in reqs:
+            r.y.fill_(float("nan"))
+        torch.randn(1 << 20, device="cuda").mul_(2.0)  # eager work between replays
+        graph.replay()
+        torch.cuda.synchronize()
+        for r, want in zip(reqs, ref):
+            assert torch.equal(r.y, want), (rep, r.kind, r.n)
+    del graph
     g.close()
 
 
@@ -189,10 +244,11 @@ def test_autotune_picks_a_gated_configuration():
     specs = group.uneven_specs(count=64, seed=5)
     reqs = group.make_requests(specs, seed=5)
     res = group.autotune(reqs, space={"workgroup": (512, 1024), "schedule": (0, 1), "order": (0, 1),
-                                      "dynamic": (0, 1), "grid": (0,)}, reps=3)
-    assert len(res.table) == 16
+                                      "dynamic": (0, 1), "grid": (0,), "twiddles": (0, 1)}, reps=3)
+    assert len(res.table) == 24  # LDS twiddles with schedule 0 only
     assert all(err <= REL_L2 for _, _, err, _, _ in res.table)
-    # schedule-0 candidates are bit-exact with the per-request launches
-    assert all(err == 0.0 for name, _, err, _, _ in res.table if "_sched0_" in name)
+    # schedule-0 candidates with device-memory twiddles are bit-exact with
+    # the per-request PLAIN launches
+    assert all(err == 0.0 for name, _, err, _, _ in res.table if "_sched0_" in name and name.endswith("_tw0"))
     assert res.best_ms == min(ms for _, ms, _, _, _ in res.table)
     torch.cuda.synchronize()

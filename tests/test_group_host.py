@@ -74,11 +74,17 @@ def test_group_unit_counts():
     assert rc == 0
     assert _query(s, h) == ((100000 + 127) // 128, 4)
     assert s.bfly_group_destroy(h) == 0
+    # LDS twiddles change where the twiddles come from, not the unit list
+    rc, h = _create(s, specs, _lib.GroupConfig(512, 0, 0, 0, 0, 1))
+    assert rc == 0
+    assert _query(s, h)[0] == 3 + 2 + 2 + 1 + 2
+    assert s.bfly_group_destroy(h) == 0
 
 
 @pytest.mark.parametrize("bad_cfg", [
     (256, 0, 0, 0, 0), (512, 2, 0, 0, 0), (512, 0, 4, 0, 0), (512, 0, -1, 0, 0),
-    (512, 0, 0, 2, 0), (512, 0, 0, 0, -5)])
+    (512, 0, 0, 2, 0), (512, 0, 0, 0, -5), (512, 0, 0, 0, 0, 2), (512, 0, 0, 0, 0, -1),
+    (512, 1, 0, 0, 0, 1)])  # LDS twiddles run radix schedule 0 only
 def test_group_config_guards(bad_cfg):
     s = bfly.stub()
     rc, h = _create(s, [(C2C, 64, 1)], _lib.GroupConfig(*bad_cfg))
