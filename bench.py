@@ -353,7 +353,7 @@ def workload_loop(name: str, steps: int, warmup: int):
         nbytes = 4 * (2 * B * Cc * Nn) + 8 * Cc * Cc * K
     elif name == "grouped":
         reqs = group.make_requests(group.uneven_specs(), seed=3)
-        grp = group.Group(reqs, group.Config(dynamic=1))
+        grp = group.Group(reqs, group.Config(order=3, dynamic=1, twiddles=1))
         graph = group.capture(grp.launch)
         fn, nbytes = graph.replay, grp.algorithmic_bytes()
     elif name in ("r2c", "c2r"):
