@@ -108,7 +108,7 @@ def cpu_baseline(budget_s: float):
     }
 
 
-def secondary_rows(cpu_seconds: float, reps: int = 20):
+def secondary_rows(cpu_seconds: float, reps: int = 50):
     """The other hot-path rows at their BASELINE shapes, inputs resident and
     rotated over 4 buffer sets (>= 256 MB MALL): cfg1 C2C latency, R2C / C2R
     4096x4096 (with the reference's refR2CF32 / refC2RF32 timed on one host
@@ -184,12 +184,15 @@ def secondary_rows(cpu_seconds: float, reps: int = 20):
                 "value": round((4 * n + 4 * (n + 2)) * done / dt / 1e9, 4), "unit": "GB/s", "cores": 1,
                 "kind": "reference", "sample": f"{done} rows of n={n} through the reference's "
                 f"{'refR2CF32' if name.startswith('r2c') else 'refC2RF32'} in {dt:.1f} s"}
-    # long C2C
+    # long C2C (library calls with their arguments prepared once)
     for n, b in ((65536, 64), (65536, 256), (16384, 1024)):
         xs = [torch.rand(b, 2 * n, device=dev) for _ in range(4)]
         ys = [torch.empty_like(xs[0]) for _ in range(4)]
         work = ops.workspace(0, n, b, dev)
-        out[f"c2c_n{n}_b{b}"] = row(timed(lambda i: ops.c2c_raw(xs[i % 4], ys[i % 4], n, b, -1, work)),
+        tab = ops.table(_lib.KIND_C2C, n, dev)
+        args = [(xs[i].data_ptr(), ys[i].data_ptr(), n, b, -1, tab.data_ptr(),
+                 work.data_ptr() if work is not None else None, sid()) for i in range(4)]
+        out[f"c2c_n{n}_b{b}"] = row(timed(lambda i: _lib.check(lib.bfly_c2c(*args[i % 4]), "long")),
                                     16 * n * b, b)
         del xs, ys, work
     return out
