@@ -495,14 +495,6 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
 
 // ------------------------------------------------------------ dispatch
 // Resident workgroups of kernel f on the current device (CUs x occupancy).
-int resident_blocks(const void* f, int block = bfly_plan::kBlock) {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, block, 0) != hipSuccess)
-        return 0;
-    return cus * (per > 0 ? per : 1);
-}
 
 template <int I, int DIR, int LX>
 hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint32_t batch,
@@ -510,8 +502,8 @@ hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint3
     constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
     const uint64_t groups = (batch + B - 1) / B;
     if (stream) {
-        static int cap = 0;
-        if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream<I, DIR, LX>));
+        static bfly_kern::ResidentCache rc;
+        const int cap = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_c2c_stream<I, DIR, LX>), bfly_plan::kBlock);
         if (cap <= 0) return hipErrorInvalidValue;
         const dim3 grid((unsigned)(groups < (uint64_t)cap ? groups : (uint64_t)cap));
         hipLaunchKernelGGL((k_c2c_stream<I, DIR, LX>), grid, dim3(bfly_plan::kBlock), 0, s, in, out, tw, batch);
@@ -527,8 +519,8 @@ hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw, const 
                          uint32_t batch, hipStream_t s) {
     constexpr int WG = lt_wg<I>(), B = WG / PlanOf<I>::T;
     const uint64_t groups = (batch + B - 1) / B;
-    static int cap = 0;
-    if (cap == 0) cap = resident_blocks(reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>), WG);
+    static bfly_kern::ResidentCache rc;
+    const int cap = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>), WG);
     if (cap <= 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)cap));
     hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(WG), 0, s, in, out, tw, tw2g, batch);
@@ -543,9 +535,8 @@ hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uin
         const uint64_t groups = (batch + B - 1) / B;
         const void* f = sign < 0 ? reinterpret_cast<const void*>(&k_c2c_stream2<I, -1>)
                                  : reinterpret_cast<const void*>(&k_c2c_stream2<I, 1>);
-        static int cap[2] = {0, 0};
-        int& c = cap[sign < 0 ? 0 : 1];
-        if (c == 0) c = resident_blocks(f);
+        static bfly_kern::ResidentCache rc[2];
+        const int c = bfly_kern::resident(rc[sign < 0 ? 0 : 1], f, bfly_plan::kBlock);
         if (c <= 0) return hipErrorInvalidValue;
         const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
         if (sign < 0)
@@ -579,9 +570,8 @@ hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, 
         if (variant == BFLY_VARIANT_STREAM_LT) {
             const void* f = kind == 1 ? reinterpret_cast<const void*>(&k_r2c_stream_lt<I>)
                                       : reinterpret_cast<const void*>(&k_c2r_stream_lt<I>);
-            static int cap[2] = {0, 0};
-            int& c = cap[kind == 1 ? 0 : 1];
-            if (c == 0) c = resident_blocks(f);
+            static bfly_kern::ResidentCache rc[2];
+            const int c = bfly_kern::resident(rc[kind == 1 ? 0 : 1], f, bfly_plan::kBlock);
             if (c <= 0) return hipErrorInvalidValue;
             const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)c));
             if (kind == 1)
@@ -597,9 +587,8 @@ hipError_t launch_real(int kind, const float* in, float* out, const float2* tw, 
     if (variant == BFLY_VARIANT_STREAM) {
         const void* f = kind == 1 ? reinterpret_cast<const void*>(&k_r2c_stream<I>)
                                   : reinterpret_cast<const void*>(&k_c2r_stream<I>);
-        static int cap[2] = {0, 0};
-        int& c = cap[kind == 1 ? 0 : 1];
-        if (c == 0) c = resident_blocks(f);
+        static bfly_kern::ResidentCache rc[2];
+        const int c = bfly_kern::resident(rc[kind == 1 ? 0 : 1], f, bfly_plan::kBlock);
         if (c <= 0) return hipErrorInvalidValue;
         const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
         if (kind == 1)
@@ -658,6 +647,23 @@ constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
 }  // namespace
 
 namespace bfly_kern {
+
+int resident(ResidentCache& cache, const void* kernel, int block) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 0;
+    std::atomic<int>* slot = dev >= 0 && dev < ResidentCache::kDevices ? &cache.per_device[dev] : nullptr;
+    if (slot) {
+        const int c = slot->load(std::memory_order_relaxed);
+        if (c > 0) return c;
+    }
+    int cus = 0, per = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, block, 0) != hipSuccess)
+        return 0;
+    const int c = cus * (per > 0 ? per : 1);
+    if (slot) slot->store(c, std::memory_order_relaxed);
+    return c;
+}
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s) {

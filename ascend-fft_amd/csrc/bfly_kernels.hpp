@@ -4,7 +4,19 @@
 #include <stddef.h>
 #include <stdint.h>
 
+#include <atomic>
+
 namespace bfly_kern {
+
+// Resident workgroups of a kernel on the current device (CUs x the
+// occupancy query), cached per device: the launchers size their persistent
+// grids with it.  Safe to share between host threads.
+struct ResidentCache {
+    static constexpr int kDevices = 64;
+    std::atomic<int> per_device[kDevices] = {};
+};
+// 0 when the runtime cannot answer.
+int resident(ResidentCache& cache, const void* kernel, int block);
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s);

@@ -352,14 +352,6 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma_stream(const flo
     }
 }
 
-int resident_blocks(const void* f) {
-    int dev = 0, cus = 0, per = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
-        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, f, bfly_plan::kBlock, 0) != hipSuccess)
-        return 0;
-    return cus * (per > 0 ? per : 1);
-}
 
 template <int I>
 hipError_t launch_mfma(const float2* in, float2* out, const float2* tw, uint32_t batch, int sign,
@@ -369,9 +361,8 @@ hipError_t launch_mfma(const float2* in, float2* out, const float2* tw, uint32_t
     if (stream) {
         const void* f = sign < 0 ? reinterpret_cast<const void*>(&k_c2c_mfma_stream<I, -1>)
                                  : reinterpret_cast<const void*>(&k_c2c_mfma_stream<I, 1>);
-        static int cap[2] = {0, 0};
-        int& c = cap[sign < 0 ? 0 : 1];
-        if (c == 0) c = resident_blocks(f);
+        static bfly_kern::ResidentCache rc[2];
+        const int c = bfly_kern::resident(rc[sign < 0 ? 0 : 1], f, bfly_plan::kBlock);
         if (c <= 0) return hipErrorInvalidValue;
         const dim3 grid((unsigned)(groups < (uint64_t)c ? groups : (uint64_t)c));
         if (sign < 0)
