@@ -333,7 +333,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream(const float* _
 
 // R2C / C2R, streaming form with LDS twiddles (the FFT's two-level tables
 // and the split twiddle W_n^k as a product of two small tables), launched
-// about 1.5 row groups per workgroup (see bfly_plan::stream_grid).
+// with the grid of bfly_plan::stream_grid.
 template <int I>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream_lt(const float* __restrict__ in,
                                                                      float* __restrict__ out,

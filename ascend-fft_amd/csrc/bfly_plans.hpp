@@ -72,20 +72,23 @@ constexpr size_t stage_twiddle_len(int i, int sched = 0) {
 
 constexpr int threads_per_transform(uint32_t n) { return (int)(n / kPoints); }
 
-// Workgroups the LDS-twiddle streaming C2C kernel launches for `groups`
-// units of work with `resident` workgroups fitting the GPU at once: about
-// 1.5 units per workgroup, never fewer workgroups than fit.  More workgroups
-// than fit means the dispatcher starts new ones as others drain, so the
-// first loads and the last computes of different workgroups overlap instead
-// of every workgroup filling and draining its prefetch pipeline at the same
+// Workgroups the LDS-twiddle streaming kernels (C2C, R2C, C2R) launch for
+// `groups` units of work with `resident` workgroups fitting the GPU at once.
+// Above two units per resident slot: about 1.5 units per workgroup.  More
+// workgroups than fit means the dispatcher starts new ones as others drain,
+// so the first loads and the last computes of different workgroups overlap
+// instead of every workgroup filling and draining its pipeline at the same
 // time (measured on MI355X, C2C 4096 x 4096: 46.6-47.9 us at 1.45-1.5 units
 // per workgroup vs 49.5 us with exactly the resident count; 4096 x 16384:
-// 173.8 vs 194 us).  The register-twiddle, real and matrix-core streaming
-// kernels measured slower this way and keep the resident count.
+// 173.8 vs 194 us; R2C / C2R 8192 x 4096: 54.7 / 57.0 vs 57.8 / 59.1 us at two
+// units per workgroup).  Up to two units per slot: the resident count (C2C
+// 4096 x 2048: 26.1 vs 27.4-28.9 us at 1.5; 8192 x 1024: 29.5 vs 32.7-33.9;
+// R2C / C2R 4096 x 4096: 28.5 / 28.0 vs 31.0 / 30.2; R2C 1024 x 16384: 32.4
+// vs 34.4).  The register-twiddle and matrix-core streaming kernels keep the
+// resident count.
 constexpr uint64_t stream_grid(uint64_t groups, uint64_t resident) {
-    uint64_t g = (groups * 2 + 2) / 3;
-    if (g < resident) g = resident;
-    return g < groups ? g : groups;
+    if (groups <= 2 * resident) return groups < resident ? groups : resident;
+    return (groups * 2 + 2) / 3;
 }
 
 // Four-step split of a long length into two local lengths N1 * N2 (column
