@@ -80,6 +80,9 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
         return g;
     if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
     if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_variant(n, batch);
+    const bool uses_work = bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant);
+    if (!bfly_host::disjoint(BFLY_KIND_C2C, in, out, uses_work ? work : nullptr, n, batch, 0, 0))
+        return BFLY_ERR_SHAPE;
     return launched(bfly_kern::c2c(in, out, n, batch, sign, variant, table, work, S(stream)));
 }
 
@@ -95,6 +98,7 @@ int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t bat
     const int sign = kind == BFLY_KIND_R2C ? -1 : 1;
     if (int g = bfly_host::check_execute(kind, sign, in, out, n, batch, table, nullptr)) return g;
     if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
+    if (!bfly_host::disjoint(kind, in, out, nullptr, n, batch, bins, stride)) return BFLY_ERR_SHAPE;
     if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_real_variant(n, batch);
     if (variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM && variant != BFLY_VARIANT_STREAM_LT)
         return BFLY_ERR_SHAPE;

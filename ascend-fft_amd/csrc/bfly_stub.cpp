@@ -88,26 +88,27 @@ int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void*) {
 }
 
 // ------------------------------------------------------------- execute
-int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
-             const float* table, void* work, void*) {
-    if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work)) return e;
-    return launch();
-}
 int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                 const float* table, void* work, void* stream) {
     (void)stream;
     if (int e = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work, variant))
         return e;
     if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_variant(n, batch);
+    const bool uses_work = bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant);
+    if (!bfly_host::disjoint(BFLY_KIND_C2C, in, out, uses_work ? work : nullptr, n, batch, 0, 0))
+        return BFLY_ERR_SHAPE;
     return launch();
 }
-int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
-    if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;
-    return launch();
+int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+             const float* table, void* work, void* stream) {
+    return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);
 }
-int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void*) {
-    if (int e = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return e;
-    return launch();
+int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* stream) {
+    return bfly_real_ex(BFLY_KIND_R2C, in, out, n, batch, n / 2 + 1, n + 2, BFLY_VARIANT_AUTO, 0, table, stream);
+}
+int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* stream) {
+    return bfly_real_ex(BFLY_KIND_C2R, in, out, n, batch, n / 2 + 1, n + 2, BFLY_VARIANT_AUTO, 0, table, stream);
 }
 int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                  uint32_t stride, int variant, uint32_t flags, const float* table, void*) {
@@ -117,22 +118,19 @@ int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t bat
                                          nullptr))
         return e;
     if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
+    if (!bfly_host::disjoint(kind, in, out, nullptr, n, batch, bins, stride)) return BFLY_ERR_SHAPE;
     if (variant != BFLY_VARIANT_AUTO && variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM &&
         variant != BFLY_VARIANT_STREAM_LT)
         return BFLY_ERR_SHAPE;
     return launch();
 }
 int bfly_r2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                uint32_t stride, const float* table, void*) {
-    if (int e = bfly_host::check_execute(BFLY_KIND_R2C, -1, in, out, n, batch, table, nullptr)) return e;
-    if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
-    return launch();
+                uint32_t stride, const float* table, void* stream) {
+    return bfly_real_ex(BFLY_KIND_R2C, in, out, n, batch, bins, stride, BFLY_VARIANT_AUTO, 0, table, stream);
 }
 int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
-                uint32_t stride, const float* table, void*) {
-    if (int e = bfly_host::check_execute(BFLY_KIND_C2R, 1, in, out, n, batch, table, nullptr)) return e;
-    if (!bfly_host::bins_ok(n, bins, stride)) return BFLY_ERR_SHAPE;
-    return launch();
+                uint32_t stride, const float* table, void* stream) {
+    return bfly_real_ex(BFLY_KIND_C2R, in, out, n, batch, bins, stride, BFLY_VARIANT_AUTO, 0, table, stream);
 }
 int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,
                            uint32_t batch, const float* table, void* work, int reps, void* stream,

@@ -142,6 +142,20 @@ static bool grid_ok(int kind, uint32_t n, uint32_t batch) {
     return blocks >= 1 && blocks <= 0x7fffffffull;
 }
 
+bool disjoint(int kind, const void* in, const void* out, const void* work, uint32_t n, uint32_t batch,
+              uint32_t bins, uint32_t stride) {
+    const uint64_t b = batch, spec = (b - 1) * stride + 2ull * bins;  // half-spectrum floats
+    const uint64_t in_f = kind == BFLY_KIND_C2C ? 2ull * n * b : kind == BFLY_KIND_R2C ? (uint64_t)n * b : spec;
+    const uint64_t out_f = kind == BFLY_KIND_C2C ? 2ull * n * b : kind == BFLY_KIND_R2C ? spec : (uint64_t)n * b;
+    auto apart = [](const void* p, uint64_t pf, const void* q, uint64_t qf) {
+        const uintptr_t a = (uintptr_t)p, c = (uintptr_t)q;
+        return a + 4 * pf <= c || c + 4 * qf <= a;
+    };
+    if (!apart(in, in_f, out, out_f)) return false;
+    if (work && (!apart(work, 2ull * n * b, in, in_f) || !apart(work, 2ull * n * b, out, out_f))) return false;
+    return true;
+}
+
 int check_execute(int kind, int sign, const void* in, const void* out, uint32_t n,
                   uint32_t batch, const void* table, const void* work, int variant) {
     if (!in || !out || batch == 0 || !shape_ok(kind, n)) return BFLY_ERR_SHAPE;
@@ -249,6 +263,7 @@ int group_plan(const bfly_group_request* reqs, int nreq, const bfly_group_config
     for (int r = 0; r < nreq; r++) {
         const bfly_group_request& q = reqs[r];
         if (!q.in || !q.out || q.batch == 0 || !group_ok(q.kind, q.n, c.workgroup)) return BFLY_ERR_SHAPE;
+        if (!disjoint(q.kind, q.in, q.out, nullptr, q.n, q.batch, q.n / 2 + 1, q.n + 2)) return BFLY_ERR_SHAPE;
         if (q.kind == BFLY_KIND_C2C && q.sign != -1 && q.sign != 1) return BFLY_ERR_SHAPE;
         P.per[r] = (uint32_t)c.workgroup / (group_local_len(q.kind, q.n) / bfly_plan::kPoints);
         P.count[r] = (uint32_t)(((uint64_t)q.batch + P.per[r] - 1) / P.per[r]);

@@ -31,6 +31,11 @@ void twiddle(uint64_t m, uint64_t L, float* out);
 size_t fill_stage(int plan, float* dst, int sched = 0);
 TableLayout table_layout(int kind, uint32_t n);
 int fill_table(int kind, uint32_t n, float* dst);
+// The buffers of one execute call do not overlap: input, output and (long
+// C2C) workspace, at the extents the call reads and writes (real kinds:
+// `bins` bins at a row stride of `stride` floats on the half-spectrum side).
+bool disjoint(int kind, const void* in, const void* out, const void* work, uint32_t n, uint32_t batch,
+              uint32_t bins, uint32_t stride);
 // Argument guards of the execute entry points: BFLY_OK or the status code.
 // `variant` matters only for long C2C, where it decides whether a workspace
 // is needed (BFLY_VARIANT_FOUR_STEP, or n above the single-workgroup limit).

@@ -143,37 +143,41 @@ def test_execute_guards_through_stub():
     s.bfly_stub_reset()
     buf = np.zeros(1 << 16, np.float32)
     p = buf.ctypes.data
-    assert s.bfly_c2c(p, p, 4096, 1, -1, p, None, None) == 0
-    assert s.bfly_c2c(p, p, 4096, 0, -1, p, None, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c(None, p, 4096, 1, -1, p, None, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c(p, p, 4096, 1, 0, p, None, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c(p, p, 4096, 1, -1, None, None, None) == _lib.ERR_TABLE
-    assert s.bfly_c2c(p, p, 32768, 1, -1, p, None, None) == bfly.ERR_SHAPE  # four-step needs work
-    assert s.bfly_c2c(p, p, 32768, 1, -1, p, p, None) == 0
-    assert s.bfly_c2c(p, p, 8192, 64, -1, p, None, None) == 0  # one workgroup per transform
-    assert s.bfly_c2c(p, p, 16384, 1, -1, p, None, None) == bfly.ERR_SHAPE  # small batch: four-step
-    assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_PLAIN, p, None, None) == 0
-    assert s.bfly_c2c_ex(p, p, 8192, 64, -1, _lib.VARIANT_FOUR_STEP, p, None, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c_ex(p, p, 8192, 1, -1, _lib.VARIANT_LAST + 1, p, None, None) == bfly.ERR_SHAPE
-    assert s.bfly_r2c(p, p, 16, 1, p, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2r(p, p, 16384, 1, p, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c(p, p, 4096, 0x7FFFFFFF, -1, p, None, None) == 0  # grid fits
-    assert s.bfly_real_ex(1, p, p, 1024, 1, 513, 1026, 0, _lib.REAL_ADJOINT, p, None) == 0
-    assert s.bfly_real_ex(2, p, p, 1024, 1, 40, 80, _lib.VARIANT_STREAM_LT, _lib.REAL_ADJOINT, p, None) == 0
-    assert s.bfly_real_ex(1, p, p, 1024, 1, 513, 1026, 0, 2, p, None) == bfly.ERR_SHAPE  # unknown flag
-    assert s.bfly_real_ex(2, p, p, 1024, 1, 513, 1026, _lib.VARIANT_STREAM2, 0, p, None) == bfly.ERR_SHAPE
-    # mode mixing: x [2,3,5], w [3,4,5], y [2,4,5] complex in one buffer, apart
-    x, w, y = p, p + 8 * 30, p + 8 * (30 + 60)
-    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y, 2, 3, 4, 5, None) == 0
-    assert s.bfly_mode_mix(_lib.MIX_GRAD_WEIGHT, x, y, w, 2, 3, 4, 5, None) == 0
-    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y - 8, 2, 3, 4, 5, None) == bfly.ERR_SHAPE  # c overlaps b
-    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y + 4, 2, 3, 4, 5, None) == bfly.ERR_SHAPE  # misaligned
-    assert s.bfly_mode_mix(3, x, w, y, 2, 3, 4, 5, None) == bfly.ERR_SHAPE
-    assert s.bfly_mode_mix(_lib.MIX_GRAD_INPUT, x, w, y, 2, 3, 4, 0, None) == bfly.ERR_SHAPE
-    assert s.bfly_mode_mix(_lib.MIX_FORWARD, x, w, y, 1 << 31, 1 << 31, 1 << 31, 1 << 31, None) == bfly.ERR_SHAPE
+    # the stub never dereferences: distinct, far-apart addresses stand for
+    # the input, output, workspace and table buffers
+    i, o, w, t = p, p + (1 << 30), p + (2 << 30), p + (3 << 30)
+    assert s.bfly_c2c(i, o, 4096, 1, -1, t, None, None) == 0
+    assert s.bfly_c2c(i, o, 4096, 0, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(None, o, 4096, 1, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(i, o, 4096, 1, 0, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(i, o, 4096, 1, -1, None, None, None) == _lib.ERR_TABLE
+    assert s.bfly_c2c(i, o, 32768, 1, -1, t, None, None) == bfly.ERR_SHAPE  # four-step needs work
+    assert s.bfly_c2c(i, o, 32768, 1, -1, t, w, None) == 0
+    assert s.bfly_c2c(i, o, 8192, 64, -1, t, None, None) == 0  # one workgroup per transform
+    assert s.bfly_c2c(i, o, 16384, 1, -1, t, None, None) == bfly.ERR_SHAPE  # small batch: four-step
+    assert s.bfly_c2c_ex(i, o, 8192, 1, -1, _lib.VARIANT_PLAIN, t, None, None) == 0
+    assert s.bfly_c2c_ex(i, o, 8192, 64, -1, _lib.VARIANT_FOUR_STEP, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_ex(i, o, 8192, 1, -1, _lib.VARIANT_LAST + 1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_r2c(i, o, 16, 1, t, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2r(i, o, 16384, 1, t, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c(i, p + (1 << 46), 4096, 0x7FFFFFFF, -1, t, None, None) == 0  # grid fits (64 TiB apart)
+    assert s.bfly_real_ex(1, i, o, 1024, 1, 513, 1026, 0, _lib.REAL_ADJOINT, t, None) == 0
+    assert s.bfly_real_ex(2, i, o, 1024, 1, 40, 80, _lib.VARIANT_STREAM_LT, _lib.REAL_ADJOINT, t, None) == 0
+    assert s.bfly_real_ex(1, i, o, 1024, 1, 513, 1026, 0, 2, t, None) == bfly.ERR_SHAPE  # unknown flag
+    assert s.bfly_real_ex(2, i, o, 1024, 1, 513, 1026, _lib.VARIANT_STREAM2, 0, t, None) == bfly.ERR_SHAPE
+    # overlapping buffers are refused at the extents each call touches
+    assert s.bfly_c2c(i, i, 4096, 1, -1, t, None, None) == bfly.ERR_SHAPE  # in place
+    assert s.bfly_c2c(i, i + 8 * 4096 - 8, 4096, 1, -1, t, None, None) == bfly.ERR_SHAPE  # one complex shared
+    assert s.bfly_c2c(i, i + 8 * 4096, 4096, 1, -1, t, None, None) == 0  # adjacent
+    assert s.bfly_c2c(i, o, 32768, 1, -1, t, o + 8, None) == bfly.ERR_SHAPE  # workspace inside out
+    assert s.bfly_c2c(i, o, 4096, 1, -1, t, o, None) == 0  # a workspace the call does not use
+    assert s.bfly_r2c(i, i + 4 * 1024, 1024, 1, t, None) == 0  # row of n reals, then the spectrum
+    assert s.bfly_r2c(i, i + 4 * 1022, 1024, 1, t, None) == bfly.ERR_SHAPE
+    assert s.bfly_real_ex(2, i, i + 4 * (3 * 80 + 2 * 40), 1024, 4, 40, 80, 0, 0, t, None) == 0  # truncated
+    assert s.bfly_real_ex(2, i, i + 4 * (3 * 80 + 2 * 40) - 4, 1024, 4, 40, 80, 0, 0, t, None) == bfly.ERR_SHAPE
     cnt = (C.c_int * 8)()
     s.bfly_stub_counters(cnt)
-    assert cnt[3] == 9  # launches counted only for accepted calls
+    assert cnt[3] == 11  # launches counted only for accepted calls
 
 
 def test_python_modules_import_and_variant_ids_match_header():

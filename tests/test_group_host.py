@@ -15,9 +15,11 @@ P = BUF.ctypes.data
 
 
 def _reqs(specs):
+    # the stub never dereferences request buffers: distinct, far-apart
+    # addresses stand for each request's input and output
     arr = (_lib.GroupRequest * len(specs))()
     for i, (kind, n, b, *sign) in enumerate(specs):
-        arr[i] = _lib.GroupRequest(P, P, kind, sign[0] if sign else -1, n, b)
+        arr[i] = _lib.GroupRequest(P + (2 * i << 36), P + ((2 * i + 1) << 36), kind, sign[0] if sign else -1, n, b)
     return arr
 
 
@@ -109,6 +111,13 @@ def test_group_null_guards():
     arr = _reqs([(C2C, 64, 1)])
     arr[0].in_ = None
     assert s.bfly_group_create(arr, 1, None, C.byref(h)) == bfly.ERR_SHAPE
+    # a request whose output overlaps its input
+    arr = _reqs([(C2C, 64, 1), (R2C, 1024, 3)])
+    arr[1].out = arr[1].in_ + 4 * 1024 * 3 - 4
+    assert s.bfly_group_create(arr, 2, None, C.byref(h)) == bfly.ERR_SHAPE
+    arr[1].out = arr[1].in_ + 4 * 1024 * 3
+    assert s.bfly_group_create(arr, 2, None, C.byref(h)) == 0 and h.value
+    assert s.bfly_group_destroy(h) == 0
     assert s.bfly_group_launch(None, None, None) == bfly.ERR_SHAPE
     assert s.bfly_group_destroy(None) == 0
 
