@@ -398,13 +398,16 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float
 //   X[k1 + N1 k2] = FFT_N2 over n2 of Y[k1][n2]                 (k_long_rows)
 // Both kernels put the C columns (rows) of one workgroup on consecutive
 // lanes, so every global access is a run of C consecutive complex values.
-// Per-column LDS regions are spaced lds_len + 1 apart to spread banks.
-template <int I1, int N2, int DIR>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
+// Per-column LDS regions are spaced lds_len + 1 apart to spread banks.  The
+// local plans here have two passes; the second pass's table is copied into
+// LDS once per workgroup (stored [r 16 + k], as stockham_lt reads it: the
+// same values as the global table, so the same bits).
+template <int I1, int N2, int DIR, int WG = bfly_plan::kBlock>
+__global__ __launch_bounds__(WG) void k_long_cols(
     const float2* __restrict__ in, float2* __restrict__ ws, const float2* __restrict__ tw1,
     const float2* __restrict__ wN) {
     using PL = PlanOf<I1>;
-    constexpr int N1 = PL::N, T = PL::T, C = bfly_plan::kBlock / T;
+    constexpr int N1 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
     constexpr int STRIDE = lds_len(N1) + 1;
     constexpr int GROUPS = N2 / C;
@@ -413,10 +416,15 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
     // entries of the full table, gathered from LDS instead of scattered
     // single-element reads of the 8N-byte table
     constexpr int NA = 256, NB = (int)(((uint64_t)N1 * N2) / NA);
+    static_assert(PL::NRAD == 2, "two-pass column plans");
+    constexpr int R1 = kPlans[I1].rad[1];
     __shared__ float2 lds[C * STRIDE];
     __shared__ float2 tab[NA + NB];
+    __shared__ float2 t1s[bfly_plan::kPoints * R1];
     const int tid = threadIdx.x;
-    for (int i = tid; i < NA + NB; i += bfly_plan::kBlock) tab[i] = i < NA ? wN[i] : wN[(i - NA) * NA];
+    for (int i = tid; i < NA + NB; i += WG) tab[i] = i < NA ? wN[i] : wN[(i - NA) * NA];
+    for (int i = tid; i < bfly_plan::kPoints * R1; i += WG)
+        t1s[(i % R1) * bfly_plan::kPoints + i / R1] = tw1[i];
     const int c = tid % C, t = tid / C;
     const uint64_t b = blockIdx.x / GROUPS;
     const int n2 = (blockIdx.x % GROUPS) * C + c;
@@ -429,7 +437,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
         for (int r = 0; r < R0; r++)
             v[m * R0 + r] = ld_stream(&src[(uint64_t)N2 * (t + m * T + r * (N1 / R0))]);
     // the first LDS exchange of the FFT orders the table writes before any read
-    fft_local<I1, T, DIR>(v, lds + c * STRIDE, tw1, t);
+    run_plan_lt<I1, T, DIR, LX_PAD>(v, lds + c * STRIDE, 0, t1s, t1s, t1s, t, std::make_index_sequence<2>{});
     float2* dst = ws + b * N + n2;
 #pragma unroll
     for (int m = 0; m < P / RL; m++)
@@ -443,25 +451,30 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_cols(
         }
 }
 
-template <int I2, int N1, int DIR>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_long_rows(
+template <int I2, int N1, int DIR, int WG = bfly_plan::kBlock>
+__global__ __launch_bounds__(WG) void k_long_rows(
     const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2) {
     using PL = PlanOf<I2>;
-    constexpr int N2 = PL::N, T = PL::T, C = bfly_plan::kBlock / T;
+    constexpr int N2 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
     constexpr int STRIDE = lds_len(N2) + 1;
     constexpr int GROUPS = N1 / C;
     static_assert(N1 % C == 0, "row groups must tile N1");
+    static_assert(PL::NRAD == 2, "two-pass row plans");
+    constexpr int R1 = kPlans[I2].rad[1];
     __shared__ float2 lds[C * STRIDE];
+    __shared__ float2 t1s[bfly_plan::kPoints * R1];
     const int tid = threadIdx.x;
+    for (int i = tid; i < bfly_plan::kPoints * R1; i += WG)
+        t1s[(i % R1) * bfly_plan::kPoints + i / R1] = tw2[i];
     const uint64_t b = blockIdx.x / GROUPS;
     const int k1_0 = (blockIdx.x % GROUPS) * C;
     constexpr uint64_t N = (uint64_t)N1 * N2;
     // Stage the C contiguous rows through LDS (row-linear, coalesced).
     const float2* src = ws + b * N + (uint64_t)k1_0 * N2;
 #pragma unroll
-    for (int i = 0; i < C * N2 / bfly_plan::kBlock; i++) {
-        const int idx = tid + i * bfly_plan::kBlock;
+    for (int i = 0; i < C * N2 / WG; i++) {
+        const int idx = tid + i * WG;
         const int row = idx / N2, col = idx % N2;
         lds[row * STRIDE + lds_idx(col)] = ld_stream(&src[idx]);
     }
@@ -474,7 +487,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_long_rows(
 #pragma unroll
         for (int r = 0; r < R0; r++) v[m * R0 + r] = L[lds_idx(t + m * T + r * (N2 / R0))];
     __syncthreads();
-    fft_local<I2, T, DIR>(v, L, tw2, t);
+    run_plan_lt<I2, T, DIR, LX_PAD>(v, L, 0, t1s, t1s, t1s, t, std::make_index_sequence<2>{});
     float2* dst = out + b * N + k1_0 + c;
 #pragma unroll
     for (int m = 0; m < P / RL; m++)
@@ -494,7 +507,6 @@ __global__ __launch_bounds__(256) void k_copy(const float4* __restrict__ src,
 }
 
 // ------------------------------------------------------------ dispatch
-// Resident workgroups of kernel f on the current device (CUs x occupancy).
 
 template <int I, int DIR, int LX>
 hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint32_t batch,
@@ -624,9 +636,22 @@ template <int I1, int I2, int DIR>
 hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const float2* tw1,
                            const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s) {
     constexpr int N1 = kPlans[I1].n, N2 = kPlans[I2].n;
-    constexpr int C1 = bfly_plan::kBlock / PlanOf<I1>::T, C2 = bfly_plan::kBlock / PlanOf<I2>::T;
-    hipLaunchKernelGGL((k_long_cols<I1, N2, DIR>), dim3(batch * (N2 / C1)), dim3(bfly_plan::kBlock),
-                       0, s, in, ws, tw1, wN);
+    // Column kernel in 512-thread workgroups when that still fills every
+    // resident slot: twice the columns per workgroup, 256-byte runs (65536 x
+    // 256: 98.8-100.9 vs 103.1-104.5 us; 65536 x 64: 25.7 vs 31.7; 32768 x
+    // 512: 95.1-96.3 vs 98.3-98.8).  With fewer column groups the 256-thread
+    // form's wider grid wins (16384 x 64: 11.5 vs 12.6 us).  The row kernel
+    // measured slower in 512-thread workgroups.
+    constexpr int C1w = 512 / PlanOf<I1>::T, C1 = bfly_plan::kBlock / PlanOf<I1>::T;
+    constexpr int C2 = bfly_plan::kBlock / PlanOf<I2>::T;
+    static bfly_kern::ResidentCache rc;
+    const int fit = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_long_cols<I1, N2, DIR, 512>), 512);
+    if (fit > 0 && (uint64_t)batch * (N2 / C1w) >= (uint64_t)fit)
+        hipLaunchKernelGGL((k_long_cols<I1, N2, DIR, 512>), dim3(batch * (N2 / C1w)), dim3(512), 0, s, in, ws,
+                           tw1, wN);
+    else
+        hipLaunchKernelGGL((k_long_cols<I1, N2, DIR>), dim3(batch * (N2 / C1)), dim3(bfly_plan::kBlock), 0, s,
+                           in, ws, tw1, wN);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),

@@ -196,8 +196,9 @@ bool Mapping::validate(const Hardware& hw, std::string* why) const {
 // LDS bytes of one 256-thread workgroup: 256/T transforms (T = n/P threads
 // each) of lds_len(n) = n + n/16 complex, i.e. 256 * P * 8 * 17/16 for every
 // n <= 4096; 8192 and 16384 hold one transform per workgroup (68 / 136
-// KiB); longer lengths take the larger of the column and row kernels
-// (C columns of lds_len(N_i) + 1 complex each).
+// KiB); longer lengths take the larger of the column kernel (512 threads
+// at most) and the row kernel (256 threads), C columns of lds_len(N_i) + 1
+// complex each.
 size_t StagePlan::ubBytes(uint32_t n) const {
     const size_t P = (size_t)std::max(radix, 2);
     auto local = [&](uint32_t m) {
@@ -207,11 +208,11 @@ size_t StagePlan::ubBytes(uint32_t n) const {
     if (n <= (uint32_t)bfly_plan::kMaxSingleN) return local(n);
     uint32_t n1 = 0, n2 = 0;
     if (!bfly_plan::long_split(n, &n1, &n2)) return local(n);
-    auto column = [&](uint32_t m) {
-        const size_t T = std::max<size_t>(1, m / P), C = std::max<size_t>(1, 256 / T);
+    auto column = [&](uint32_t m, size_t threads) {
+        const size_t T = std::max<size_t>(1, m / P), C = std::max<size_t>(1, threads / T);
         return C * ((size_t)m + m / 16 + 1) * 8;
     };
-    return std::max(column(n1), column(n2));
+    return std::max(column(n1, 512), column(n2, 256));
 }
 
 // ==================================================================== Q

@@ -244,10 +244,15 @@ inline KernelResource gfx_stage_resource(uint32_t len, int stages, int stage) {
         return KernelResource{"k_c2c_single", CoreKind::AIVVectorCore, (size_t)(gfx_lds_len(len) * 8),
                               SyncScope::AIVIntraCore, "n/16 threads, one transform, padded LDS"};
     }
-    const uint64_t cols = kGfxBlock / (len / kGfxPoints);
+    // the column kernel runs 512-thread workgroups (256 when the batch leaves
+    // fewer column groups than fit the GPU: the figure here is the larger),
+    // the row kernel 256
+    const uint64_t threads = stage == 0 ? 2ull * kGfxBlock : kGfxBlock;
+    const uint64_t cols = threads / (len / kGfxPoints);
     return KernelResource{stage == 0 ? "k_long_cols" : "k_long_rows", CoreKind::AIVVectorCore,
                           (size_t)(cols * (gfx_lds_len(len) + 1) * 8), SyncScope::AIVIntraCore,
-                          "256 threads, 256/(len/16) columns, padded LDS regions + 1"};
+                          stage == 0 ? "512 threads, 512/(len/16) columns, padded LDS regions + 1"
+                                     : "256 threads, 256/(len/16) rows, padded LDS regions + 1"};
 }
 
 // Two tensor passes (read + write) per launch, the reference's payload

@@ -78,7 +78,8 @@ def test_runtime_lowering_of_every_long_and_local_length(harness):
                 n1 = 1 << (k // 2)
                 n2 = n // n1
                 cases.append(_case(n=n, batch=b, n1=n1, n2=n2))
-                lds = max((256 // (m // 16)) * (_lds_len(m) + 1) * 8 for m in (n1, n2))
+                # column kernel: 512 threads; row kernel: 256
+                lds = max((512 // (n1 // 16)) * (_lds_len(n1) + 1) * 8, (256 // (n2 // 16)) * (_lds_len(n2) + 1) * 8)
                 want.append((1, 2, 1, 0, 2 * 2 * 8 * n * b, "row_fft|row_fft", lds))
     for r, (sup, launches, edges, on_chip, payload, kinds, lds) in zip(_run(harness, cases), want):
         assert r["supported"] == sup and r["abstract_feasible"] == 1, r
