@@ -324,6 +324,59 @@ def spectral_pattern(reps: int = 10):
     }
 
 
+def grid49_pattern(reps: int = 50):
+    """The plain drop-in grid of 49 points, C2C forward fp32, n in
+    {64..4096} x batch in {1,4,...,4096} (inputs U[-1,1) from seed 1): the
+    library (AUTO kernel) against torch.fft.fft (hipFFT) on the same GPU and
+    the same device-resident complex64 tensors, both launched back to back
+    from Python and timed with HIP events; geometric-mean speedup and wins, as
+    the reference reports its grid against its vendor FFT."""
+    import numpy as np
+    import torch
+    from bfly import _lib, ops
+
+    lib = _lib.hip()
+    dev = torch.device("cuda:0")
+    sid = torch.cuda.current_stream().cuda_stream
+    g = torch.Generator(device=dev)
+    g.manual_seed(1)
+
+    def timed(fn):
+        for _ in range(5):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    points, ratios = [], []
+    for n in (64, 128, 256, 512, 1024, 2048, 4096):
+        tab = ops.table(_lib.KIND_C2C, n, dev)
+        for b in (1, 4, 16, 64, 256, 1024, 4096):
+            x = torch.complex(torch.rand(b, n, generator=g, device=dev) * 2 - 1,
+                              torch.rand(b, n, generator=g, device=dev) * 2 - 1)
+            y = torch.empty_like(x)
+            args = (x.data_ptr(), y.data_ptr(), n, b, -1, tab.data_ptr(), None, sid)
+            ours = timed(lambda: lib.bfly_c2c(*args))
+            ref = timed(lambda: torch.fft.fft(x))
+            err = float((y - torch.fft.fft(x)).abs().max() / torch.fft.fft(x).abs().max())
+            points.append({"n": n, "batch": b, "us": round(ours, 2), "us_torch_fft": round(ref, 2),
+                           "speedup": round(ref / ours, 3), "max_rel_diff": err})
+            ratios.append(ref / ours)
+            del x, y
+    return {
+        "workload": "C2C fp32 grid n 64..4096 x batch 1..4096 (49 points), seed 1, vs torch.fft.fft (hipFFT)",
+        "geomean_speedup_vs_torch_fft": round(float(np.exp(np.mean(np.log(ratios)))), 3),
+        "wins": int(sum(r > 1.0 for r in ratios)),
+        "losses": int(sum(r <= 1.0 for r in ratios)),
+        "points": points,
+    }
+
+
 def workload_loop(name: str, steps: int, warmup: int):
     """One secondary workload run `steps` times after `warmup` untimed runs,
     for kernel traces and counter passes of its kernels (--only NAME).
@@ -504,7 +557,8 @@ def main():
     }
     if not args.no_patterns:
         line["rows"] = secondary_rows(0 if args.no_cpu else 1.5)
-        line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern()}
+        line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern(),
+                            "grid49_vs_torch_fft": grid49_pattern()}
     print(json.dumps(line))
 
 
