@@ -144,10 +144,11 @@ __device__ __forceinline__ void dft16_mfma(float2 (&v)[16], const half8 (&ahi)[2
 }
 
 // One radix-16 MFMA pass with Ns points already combined.  In: the lane's
-// inputs in v (see dft16_mfma); multiplies by the pass twiddles (Ns > 1),
-// runs the DFT and, unless LAST, writes the outputs to `lds` in Stockham
-// order.  The caller synchronises.
-template <int N, int Ns, int DIR, bool LAST>
+// inputs in v (see dft16_mfma); multiplies by the pass twiddles (Ns > 1;
+// entry [k][q] of the pass table at tw[k PITCH + q]), runs the DFT and,
+// unless LAST, writes the outputs to `lds` in Stockham order.  The caller
+// synchronises.
+template <int N, int Ns, int DIR, bool LAST, int PITCH = 16>
 __device__ __forceinline__ void pass16_mfma(float2 (&v)[16], float2* lds, const float2* __restrict__ tw,
                                             int w, int lane, const half8 (&ahi)[2], const half8 (&alo)[2]) {
     const int h = lane >> 4, c = lane & 15;
@@ -156,7 +157,7 @@ __device__ __forceinline__ void pass16_mfma(float2 (&v)[16], float2* lds, const 
         for (int i = 0; i < 4; i++) {
             const int j = 64 * w + 16 * i + c;
             const int k = j & (Ns - 1);
-            const float4* w4 = reinterpret_cast<const float4*>(tw + k * 16 + 4 * h);
+            const float4* w4 = reinterpret_cast<const float4*>(tw + k * PITCH + 4 * h);
 #pragma unroll
             for (int q2 = 0; q2 < 2; q2++) {
                 const float4 t = w4[q2];
@@ -219,12 +220,14 @@ __device__ __forceinline__ void load16_global(float2 (&v)[16], const float2* __r
 // One transform of plan [16, 16, R2] whose pass-0 inputs are in v: two MFMA
 // passes, then radix R2 -- on the matrix cores when R2 = 16, on the vector
 // ALUs otherwise -- and the natural-order store to dst (when active).
-// `lds` is the transform's region, `s_exp` its per-wave exponent slots.
-template <int I, int DIR>
+// `lds` is the transform's region, `s_exp` its per-wave exponent slots; the
+// second pass reads its table from tw1 at a row pitch of PITCH1 (tw itself,
+// or a copy of those 256 entries, same values, in LDS).
+template <int I, int DIR, int PITCH1 = 16>
 __device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int* s_exp,
-                                               const float2* __restrict__ tw, const half8 (&ahi)[2],
-                                               const half8 (&alo)[2], int t, float2* __restrict__ dst,
-                                               bool active) {
+                                               const float2* __restrict__ tw, const float2* tw1,
+                                               const half8 (&ahi)[2], const half8 (&alo)[2], int t,
+                                               float2* __restrict__ dst, bool active) {
     using PL = PlanOf<I>;
     constexpr int N = PL::N, T = PL::T;
     constexpr int R2 = bfly_plan::kPlans[I].rad[2];
@@ -253,7 +256,7 @@ __device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int
     read16_lds<N>(v, lds, w, lane);
     __syncthreads();
     // pass 1: Ns = 16, table of the pass at tw[0 .. 256)
-    pass16_mfma<N, 16, DIR, false>(v, lds, tw, w, lane, ahi, alo);
+    pass16_mfma<N, 16, DIR, false, PITCH1>(v, lds, tw1, w, lane, ahi, alo);
     __syncthreads();
     const float2* tw2 = tw + 16 * 16;
     if constexpr (R2 == 16) {
@@ -311,7 +314,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
     dft16_operands<DIR>(t & 63, ahi, alo);
     float2 v[16];
     load16_global<N>(v, in + tr * N, active, t >> 6, t & 63);
-    mfma_transform<I, DIR>(v, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, ahi, alo, t, out + tr * N,
+    mfma_transform<I, DIR>(v, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, tw, ahi, alo, t, out + tr * N,
                            active);
 }
 
@@ -323,9 +326,16 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma_stream(const flo
                                                                      const float2* __restrict__ tw,
                                                                      uint32_t batch) {
     constexpr int N = PlanOf<I>::N, T = PlanOf<I>::T, B = bfly_plan::kBlock / T, WAVES = T / 64;
+    // the second pass's table copied once per workgroup, rows of 16 entries
+    // at a pitch of 18: a lane group's 16-byte reads then hit distinct banks
+    constexpr int PITCH1 = 18;
     __shared__ float2 lds_all[B * lds_len(N)];
     __shared__ int s_exp[B * WAVES];
+    __shared__ float4 tw1s4[16 * PITCH1 / 2];
+    float2* tw1s = reinterpret_cast<float2*>(tw1s4);
     const int tid = threadIdx.x;
+    for (int i = tid; i < 256; i += bfly_plan::kBlock) tw1s[(i / 16) * PITCH1 + i % 16] = tw[i];
+    // (the transform's first LDS barrier orders these writes before the reads)
     const int bl = tid / T, t0 = tid % T;
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
     half8 ahi[2], alo[2];
@@ -345,8 +355,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma_stream(const flo
             load16_global<N>(nxt, in + trn * N, trn < batch, t >> 6, t & 63);
         }
         const uint64_t tr = (uint64_t)g * B + bl;
-        mfma_transform<I, DIR>(cur, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, ahi, alo, t,
-                               out + tr * N, tr < batch);
+        mfma_transform<I, DIR, PITCH1>(cur, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, tw1s, ahi, alo,
+                                       t, out + tr * N, tr < batch);
 #pragma unroll
         for (int i = 0; i < 16; i++) cur[i] = nxt[i];
     }
