@@ -59,7 +59,9 @@ ERRORS = {
 SYMBOLS = (
     "bfly_rt_init", "bfly_rt_stream_create", "bfly_rt_stream_destroy", "bfly_rt_stream_sync",
     "bfly_rt_malloc", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
-    "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
+    "bfly_rt_memcpy_h2d_async", "bfly_rt_memcpy_d2h_async", "bfly_rt_malloc_host", "bfly_rt_free_host",
+    "bfly_rt_event_create", "bfly_rt_event_destroy", "bfly_rt_event_record", "bfly_rt_event_elapsed",
+    "bfly_c2c_launches", "bfly_c2c_traced", "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
     "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
     "bfly_group_query", "bfly_group_destroy", "bfly_mode_mix", "bfly_probe_copy", "bfly_probe_device",
@@ -95,6 +97,16 @@ def _bind(lib):
         "bfly_rt_free": (i32, [vp]),
         "bfly_rt_memcpy_h2d": (i32, [vp, vp, sz, vp]),
         "bfly_rt_memcpy_d2h": (i32, [vp, vp, sz, vp]),
+        "bfly_rt_memcpy_h2d_async": (i32, [vp, vp, sz, vp]),
+        "bfly_rt_memcpy_d2h_async": (i32, [vp, vp, sz, vp]),
+        "bfly_rt_malloc_host": (i32, [C.POINTER(vp), sz]),
+        "bfly_rt_free_host": (i32, [vp]),
+        "bfly_rt_event_create": (i32, [C.POINTER(vp)]),
+        "bfly_rt_event_destroy": (i32, [vp]),
+        "bfly_rt_event_record": (i32, [vp, vp]),
+        "bfly_rt_event_elapsed": (i32, [vp, vp, C.POINTER(C.c_float)]),
+        "bfly_c2c_launches": (i32, [u32, u32, i32]),
+        "bfly_c2c_traced": (i32, [vp, vp, u32, u32, i32, i32, vp, vp, vp, C.POINTER(vp), i32]),
         "bfly_rt_time_transform": (i32, [i32, i32, vp, vp, u32, u32, vp, vp, i32, vp,
                                          C.POINTER(C.c_float)]),
         "bfly_supported": (i32, [i32, u32]),

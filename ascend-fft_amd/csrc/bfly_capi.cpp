@@ -74,6 +74,86 @@ int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
     return hipStreamSynchronize(S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_D2H;
 }
 
+int bfly_rt_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return BFLY_ERR_H2D;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)) == hipSuccess ? BFLY_OK
+                                                                                          : BFLY_ERR_H2D;
+}
+
+int bfly_rt_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream) {
+    if (!dst || !src) return BFLY_ERR_D2H;
+    return hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)) == hipSuccess ? BFLY_OK
+                                                                                          : BFLY_ERR_D2H;
+}
+
+int bfly_rt_malloc_host(void** ptr, size_t bytes) {
+    if (!ptr) return BFLY_ERR_SHAPE;
+    *ptr = nullptr;
+    if (hipHostMalloc(ptr, bytes ? bytes : 16, hipHostMallocDefault) != hipSuccess) {
+        *ptr = nullptr;
+        return BFLY_ERR_ALLOC;
+    }
+    return BFLY_OK;
+}
+
+int bfly_rt_free_host(void* ptr) {
+    if (!ptr) return BFLY_OK;
+    return hipHostFree(ptr) == hipSuccess ? BFLY_OK : BFLY_ERR_ALLOC;
+}
+
+int bfly_rt_event_create(void** event) {
+    if (!event) return BFLY_ERR_SHAPE;
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return BFLY_ERR_ALLOC;
+    *event = e;
+    return BFLY_OK;
+}
+
+int bfly_rt_event_destroy(void* event) {
+    if (!event) return BFLY_OK;
+    return hipEventDestroy(reinterpret_cast<hipEvent_t>(event)) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
+}
+
+int bfly_rt_event_record(void* event, void* stream) {
+    if (!event) return BFLY_ERR_SHAPE;
+    return hipEventRecord(reinterpret_cast<hipEvent_t>(event), S(stream)) == hipSuccess ? BFLY_OK
+                                                                                         : BFLY_ERR_LAUNCH;
+}
+
+int bfly_rt_event_elapsed(void* start, void* end, float* ms) {
+    if (!start || !end || !ms) return BFLY_ERR_SHAPE;
+    hipEvent_t a = reinterpret_cast<hipEvent_t>(start), b = reinterpret_cast<hipEvent_t>(end);
+    if (hipEventSynchronize(b) != hipSuccess) return BFLY_ERR_SYNC;
+    return hipEventElapsedTime(ms, a, b) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
+}
+
+int bfly_c2c_launches(uint32_t n, uint32_t batch, int variant) {
+    if (!bfly_supported(BFLY_KIND_C2C, n) || batch == 0) return BFLY_ERR_SHAPE;
+    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    return bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant) ? 2 : 1;
+}
+
+int bfly_c2c_traced(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
+                    const float* table, void* work, void* stream, void* const* events, int nevents) {
+    const int launches = bfly_c2c_launches(n, batch, variant);
+    if (launches < 0) return launches;
+    if (!events || nevents != launches + 1) return BFLY_ERR_SHAPE;
+    for (int i = 0; i < nevents; i++)
+        if (!events[i]) return BFLY_ERR_SHAPE;
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work, variant))
+        return g;
+    if (variant == BFLY_VARIANT_AUTO) variant = bfly_host::auto_variant(n, batch);
+    const bool uses_work = launches == 2;
+    if (!bfly_host::disjoint(BFLY_KIND_C2C, in, out, uses_work ? work : nullptr, n, batch, 0, 0))
+        return BFLY_ERR_SHAPE;
+    auto ev = [&](int i) { return reinterpret_cast<hipEvent_t>(events[i]); };
+    if (hipEventRecord(ev(0), S(stream)) != hipSuccess) return BFLY_ERR_LAUNCH;
+    const int rc = launched(bfly_kern::c2c(in, out, n, batch, sign, variant, table, work, S(stream),
+                                           launches == 2 ? ev(1) : nullptr));
+    if (rc != BFLY_OK) return rc;
+    return hipEventRecord(ev(launches), S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH;
+}
+
 int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                 const float* table, void* work, void* stream) {
     if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work, variant))

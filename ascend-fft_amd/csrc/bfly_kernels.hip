@@ -634,7 +634,8 @@ hipError_t launch_c2c_single(const float2* in, float2* out, const float2* tw, ui
 
 template <int I1, int I2, int DIR>
 hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const float2* tw1,
-                           const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s) {
+                           const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s,
+                           hipEvent_t mid) {
     constexpr int N1 = kPlans[I1].n, N2 = kPlans[I2].n;
     // Column kernel in 512-thread workgroups when that still fills every
     // resident slot: twice the columns per workgroup, 256-byte runs (65536 x
@@ -654,6 +655,7 @@ hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const floa
                            in, ws, tw1, wN);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
+    if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),
                        0, s, ws, out, tw2);
     return hipGetLastError();
@@ -662,9 +664,9 @@ hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const floa
 template <int I1, int I2>
 hipError_t launch_long(const float2* in, float2* out, float2* ws, const float2* tw1,
                        const float2* tw2, const float2* wN, uint32_t batch, int sign,
-                       hipStream_t s) {
-    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s)
-                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s);
+                       hipStream_t s, hipEvent_t mid) {
+    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s, mid)
+                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s, mid);
 }
 
 constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
@@ -691,7 +693,7 @@ int resident(ResidentCache& cache, const void* kernel, int block) {
 }
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
-               const float* table, void* work, hipStream_t s) {
+               const float* table, void* work, hipStream_t s, hipEvent_t mid) {
     if (variant == BFLY_VARIANT_SPLIT_MFMA || variant == BFLY_VARIANT_SPLIT_MFMA_STREAM)
         return c2c_mfma(in, out, n, batch, sign, variant == BFLY_VARIANT_SPLIT_MFMA_STREAM, table, s);
     const float2* i2 = reinterpret_cast<const float2*>(in);
@@ -732,10 +734,10 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
     const float2 *t1 = tw + L.stage0, *t2 = tw + L.stage1, *wN = tw + L.extra;
     float2* ws = reinterpret_cast<float2*>(work);
     switch (n) {
-        case 8192: return launch_long<idx(64), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
-        case 16384: return launch_long<idx(128), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
-        case 32768: return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
-        case 65536: return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s);
+        case 8192: return launch_long<idx(64), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
+        case 16384: return launch_long<idx(128), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
+        case 32768: return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
+        case 65536: return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
         default: return hipErrorInvalidValue;
     }
 }

@@ -6,6 +6,7 @@
 // Context/Plan guard and cache-invalidation tests run without a GPU.
 #include <cstdint>
 #include <cstdio>
+#include <chrono>
 #include <cstdlib>
 #include <cstring>
 #include <new>
@@ -87,7 +88,58 @@ int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void*) {
     return BFLY_OK;
 }
 
+int bfly_rt_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream) {
+    return bfly_rt_memcpy_h2d(dst, src, bytes, stream);
+}
+int bfly_rt_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream) {
+    return bfly_rt_memcpy_d2h(dst, src, bytes, stream);
+}
+int bfly_rt_malloc_host(void** ptr, size_t bytes) { return bfly_rt_malloc(ptr, bytes); }
+int bfly_rt_free_host(void* ptr) { return bfly_rt_free(ptr); }
+// events hold the host clock at their last record (ms)
+int bfly_rt_event_create(void** event) {
+    if (!event) return BFLY_ERR_SHAPE;
+    *event = new double(0.0);
+    return BFLY_OK;
+}
+int bfly_rt_event_destroy(void* event) {
+    delete static_cast<double*>(event);
+    return BFLY_OK;
+}
+int bfly_rt_event_record(void* event, void*) {
+    if (!event) return BFLY_ERR_SHAPE;
+    *static_cast<double*>(event) =
+        std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now().time_since_epoch()).count();
+    return BFLY_OK;
+}
+int bfly_rt_event_elapsed(void* start, void* end, float* ms) {
+    if (!start || !end || !ms) return BFLY_ERR_SHAPE;
+    *ms = (float)(*static_cast<double*>(end) - *static_cast<double*>(start));
+    return BFLY_OK;
+}
+
 // ------------------------------------------------------------- execute
+int bfly_c2c_launches(uint32_t n, uint32_t batch, int variant) {
+    if (!bfly_supported(BFLY_KIND_C2C, n) || batch == 0) return BFLY_ERR_SHAPE;
+    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    return bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant) ? 2 : 1;
+}
+int bfly_c2c_traced(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
+                    const float* table, void* work, void* stream, void* const* events, int nevents) {
+    const int launches = bfly_c2c_launches(n, batch, variant);
+    if (launches < 0) return launches;
+    if (!events || nevents != launches + 1) return BFLY_ERR_SHAPE;
+    for (int i = 0; i < nevents; i++)
+        if (!events[i]) return BFLY_ERR_SHAPE;
+    // events change only when the call is accepted
+    double start = 0.0;
+    (void)bfly_rt_event_record(&start, stream);
+    const int rc = bfly_c2c_ex(in, out, n, batch, sign, variant, table, work, stream);
+    if (rc != BFLY_OK) return rc;
+    *static_cast<double*>(events[0]) = start;
+    for (int i = 1; i < nevents; i++) (void)bfly_rt_event_record(events[i], stream);
+    return BFLY_OK;
+}
 int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                 const float* table, void* work, void* stream) {
     (void)stream;

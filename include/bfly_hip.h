@@ -63,6 +63,22 @@ BFLY_API int bfly_rt_malloc(void** ptr, size_t bytes);
 BFLY_API int bfly_rt_free(void* ptr);
 BFLY_API int bfly_rt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 BFLY_API int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+/* The same two copies without the stream synchronisation (aclrtMemcpyAsync
+ * of the reference's long-FFT device chain and end-to-end loop,
+ * src/host/fft_check.cpp:853, 877, 1043-1047). */
+BFLY_API int bfly_rt_memcpy_h2d_async(void* dst, const void* src, size_t bytes, void* stream);
+BFLY_API int bfly_rt_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* stream);
+/* Page-locked host memory (aclrtMallocHost / aclrtFreeHost of the
+ * end-to-end loop, src/host/fft_check.cpp:1023-1034). */
+BFLY_API int bfly_rt_malloc_host(void** ptr, size_t bytes);
+BFLY_API int bfly_rt_free_host(void* ptr);
+/* Stream events (aclrtCreateEvent / aclrtRecordEvent / aclrtEventElapsedTime
+ * of the device-chain timing, src/host/fft_check.cpp:823-830, 852-896).
+ * bfly_rt_event_elapsed waits for `end`, then gives end - start in ms. */
+BFLY_API int bfly_rt_event_create(void** event);
+BFLY_API int bfly_rt_event_destroy(void* event);
+BFLY_API int bfly_rt_event_record(void* event, void* stream);
+BFLY_API int bfly_rt_event_elapsed(void* start, void* end, float* ms);
 /* Elapsed milliseconds of `reps` back-to-back launches of one transform on
  * `stream`, measured with HIP events (replaces the host wall-clock loop of
  * Plan::measure, butterfly.cpp:852-924). */
@@ -104,6 +120,20 @@ BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
  * butterfly.cpp:625, src/ascendc/fft_radix2.cpp:64-69). */
 BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream);
+
+/* Kernel launches bfly_c2c_ex makes for (n, batch, variant): 1, or 2 for
+ * the four-step chain (the column kernel -- transpose in, N1-point FFTs,
+ * twiddle -- then the row kernel -- N2-point FFTs, natural-order store);
+ * BFLY_ERR_SHAPE for an unsupported length, batch 0 or an unknown variant. */
+BFLY_API int bfly_c2c_launches(uint32_t n, uint32_t batch, int variant);
+/* bfly_c2c_ex with its launches bracketed by events, for per-segment timing
+ * (the evSeg markers of the reference's device chain,
+ * src/host/fft_check.cpp:852-896): events[0] is recorded before the first
+ * launch and events[i] after launch i, so nevents must be
+ * bfly_c2c_launches(n, batch, variant) + 1. */
+BFLY_API int bfly_c2c_traced(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+                             int variant, const float* table, void* work, void* stream,
+                             void* const* events, int nevents);
 
 /* Kernel variants of one C2C length (same table, same results):
  *   BFLY_VARIANT_AUTO    the library's choice for (n, batch)

@@ -226,6 +226,13 @@ def app_input(n: int, batch: int, mode: str) -> np.ndarray:
     return out
 
 
+def sin_input(count: int) -> np.ndarray:
+    """The `sin` / `deterministic-sin-cos` sequence over a flat float index."""
+    out = np.empty(count, np.float32)
+    lib().orc_gen_sin(out, count)
+    return out
+
+
 def half_input(n: int, batch: int) -> np.ndarray:
     out = np.zeros((batch, n + 2), np.float32)
     lib().orc_gen_half(out, n, batch, n + 2)

@@ -180,6 +180,40 @@ def test_execute_guards_through_stub():
     assert cnt[3] == 11  # launches counted only for accepted calls
 
 
+def test_chain_launch_count_and_traced_guards_through_stub():
+    s = bfly.stub()
+    # one launch up to 4096, one workgroup per transform for 8192 / 16384
+    # from 256 rows (any batch at 8192), the two-launch four-step otherwise
+    for n, b, v, want in ((4096, 4096, 0, 1), (8192, 1, 0, 1), (16384, 256, 0, 1), (16384, 255, 0, 2),
+                          (32768, 1, 0, 2), (65536, 256, 0, 2), (8192, 64, _lib.VARIANT_FOUR_STEP, 2),
+                          (4096, 16, _lib.VARIANT_SPLIT_MFMA, 1)):
+        assert s.bfly_c2c_launches(n, b, v) == want, (n, b, v)
+    assert s.bfly_c2c_launches(100, 1, 0) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_launches(4096, 0, 0) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_launches(4096, 1, _lib.VARIANT_LAST + 1) == bfly.ERR_SHAPE
+    s.bfly_stub_reset()
+    buf = np.zeros(1 << 16, np.float32)
+    p = buf.ctypes.data
+    i, o, w, t = p, p + (1 << 30), p + (2 << 30), p + (3 << 30)
+    ev = [C.c_void_p() for _ in range(3)]
+    for e in ev:
+        assert s.bfly_rt_event_create(C.byref(e)) == 0
+    arr = (C.c_void_p * 3)(*[e.value for e in ev])
+    assert s.bfly_c2c_traced(i, o, 4096, 2, -1, 0, t, None, None, arr, 2) == 0
+    assert s.bfly_c2c_traced(i, o, 4096, 2, -1, 0, t, None, None, arr, 3) == bfly.ERR_SHAPE  # one launch
+    assert s.bfly_c2c_traced(i, o, 65536, 2, -1, 0, t, w, None, arr, 3) == 0
+    assert s.bfly_c2c_traced(i, o, 65536, 2, -1, 0, t, None, None, arr, 3) == bfly.ERR_SHAPE  # no workspace
+    assert s.bfly_c2c_traced(i, i, 4096, 2, -1, 0, t, None, None, arr, 2) == bfly.ERR_SHAPE  # in place
+    assert s.bfly_c2c_traced(i, o, 4096, 2, -1, 0, t, None, None, None, 2) == bfly.ERR_SHAPE
+    ms = C.c_float()
+    assert s.bfly_rt_event_elapsed(arr[0], arr[1], C.byref(ms)) == 0 and ms.value >= 0.0
+    for e in ev:
+        assert s.bfly_rt_event_destroy(e) == 0
+    cnt = (C.c_int * 8)()
+    s.bfly_stub_counters(cnt)
+    assert cnt[3] == 2
+
+
 def test_python_modules_import_and_variant_ids_match_header():
     import importlib
     for m in ("bfly.ops", "bfly.tune", "bfly.autograd", "bfly.spectral"):

@@ -1,0 +1,208 @@
+"""CPU: the bfly_check program (ascend-fft_amd/csrc/check/) against the C
+ABI's CPU stub, and the report parser bfly.scopes.
+
+The stub runs every guard but computes nothing, which makes it the planted
+mistake for the program's own checks: every transform must FAIL and a
+sequence of different inputs must be flagged STALE-OUTPUT.  The generated
+inputs (dumped with AB_DUMP) must equal the oracle's restatement of the
+reference's fft_check generators bit for bit.  Argument handling follows
+src/host/fft_check.cpp:213-300 (exit 2 on a bad argument).
+"""
+import os
+import shutil
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+import oracle
+from bfly import scopes
+
+ROOT = Path(__file__).resolve().parents[1]
+PKG = ROOT / "ascend-fft_amd"
+
+
+@pytest.fixture(scope="module")
+def cli(tmp_path_factory):
+    if not shutil.which("g++"):
+        pytest.skip("needs g++")
+    stub = PKG / "lib" / "libbfly_stub.so"
+    assert stub.exists(), "build first: make -C ascend-fft_amd"
+    exe = tmp_path_factory.mktemp("cli") / "bfly_check_stub"
+    subprocess.run(
+        ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-fsanitize=address,undefined",
+         f"-I{ROOT / 'include'}", f"-I{PKG / 'csrc'}", f"-I{PKG / 'csrc' / 'check'}",
+         str(PKG / "csrc" / "check" / "bfly_check.cpp"), str(PKG / "csrc" / "check" / "inputs.cpp"),
+         str(PKG / "csrc" / "framework" / "reference.cpp"),
+         f"-L{stub.parent}", "-lbfly_stub", f"-Wl,-rpath,{stub.parent}", "-o", str(exe)],
+        check=True, capture_output=True, text=True)
+    return exe
+
+
+def run(exe, *args, **envs):
+    env = {k: v for k, v in os.environ.items() if not k.startswith("AB_")}
+    env.update({k: str(v) for k, v in envs.items()})
+    return subprocess.run([str(exe), *map(str, args)], capture_output=True, text=True, env=env,
+                          timeout=120)
+
+
+@pytest.mark.parametrize("args,envs,msg", [
+    ((), {}, "usage"),
+    ((100, 2), {}, "n must be power of two"),
+    ((4, 2), {}, "n must be power of two"),
+    ((64, 2), {"AB_DIR": "fft"}, "AB_DIR must be"),
+    ((64, 2), {"AB_INPUT": "square"}, "unsupported AB_INPUT mode"),
+    ((64, 2), {"AB_PLANE_K": "12"}, "AB_PLANE_K must be one of 8|16|32"),
+    ((256, 2), {"AB_DIR": "r2c", "AB_INPUT_SEQ": "sin,zero"}, "only supported for c2c"),
+    ((256, 2), {"AB_INPUT_SEQ": "sin"}, "at least two inputs"),
+    ((131072, 1), {}, "outside the library's envelope"),
+    ((16384, 1), {"AB_DIR": "c2r"}, "outside the library's envelope"),
+    ((8192, 2), {"AB_BOUNDARY": "host"}, "AB_BOUNDARY must be device"),
+    ((8192, 2), {"AB_LONG_BOUNDARY_IMPL": "separate"}, "AB_LONG_BOUNDARY_IMPL must be fused"),
+    ((256, 2), {"AB_DIR": "r2c", "AB_SGN": "+1"}, "fixed sign"),
+    ((256, 2), {"AB_VARIANT": "fastest"}, "unknown AB_VARIANT"),
+    ((256, 2), {"AB_INPUT_FILE": "/nonexistent/in.bin"}, "AB_INPUT_FILE read failed"),
+])
+def test_bad_arguments_exit_2(cli, args, envs, msg):
+    r = run(cli, *args, **envs)
+    assert r.returncode == 2, r.stdout + r.stderr
+    assert msg in r.stdout
+
+
+def _dump(cli, tmp_path, n, b, **envs):
+    prefix = tmp_path / "d"
+    r = run(cli, n, b, 1, AB_DUMP=prefix, **envs)
+    # the stub computes nothing: the verdict must be FAIL
+    assert r.returncode == 1 and r.stdout.rstrip().endswith("FAIL"), r.stdout + r.stderr
+    return np.fromfile(f"{prefix}.in.bin", np.float32)
+
+
+@pytest.mark.parametrize("mode,alias", [
+    ("zero", None), ("impulse", None), ("constant", None), ("tone", "single-tone"),
+    ("alternating", "nyquist-alternating"), ("random-seeded", "random"),
+    ("high-dynamic-range", "dynamic"), ("near-cancellation", None)])
+def test_numerical_inputs_match_the_oracle(cli, tmp_path, mode, alias):
+    n, b = 64, 9
+    want = oracle.numerical_input(n, b, mode, seed=0x1234).reshape(-1)
+    for name in (mode, alias) if alias else (mode,):
+        got = _dump(cli, tmp_path, n, b, AB_INPUT=name, AB_SEED="0x1234")
+        np.testing.assert_array_equal(got.view(np.uint32), want.view(np.uint32))
+
+
+@pytest.mark.parametrize("mode", ["ofdm", "radar", "dl"])
+def test_application_inputs_match_the_oracle(cli, tmp_path, mode):
+    got = _dump(cli, tmp_path, 256, 5, AB_INPUT=mode)
+    np.testing.assert_array_equal(got.view(np.uint32), oracle.app_input(256, 5, mode).reshape(-1).view(np.uint32))
+
+
+def test_sin_inputs_and_real_directions_match_the_oracle(cli, tmp_path):
+    got = _dump(cli, tmp_path, 128, 3)
+    np.testing.assert_array_equal(got, oracle.sin_input(2 * 128 * 3))
+    got = _dump(cli, tmp_path, 128, 3, AB_DIR="r2c")
+    np.testing.assert_array_equal(got, oracle.sin_input(128 * 3))
+    # R2C of a complex mode takes the real parts
+    got = _dump(cli, tmp_path, 128, 3, AB_DIR="r2c", AB_INPUT="random-seeded", AB_SEED=5)
+    np.testing.assert_array_equal(got, oracle.numerical_input(128, 3, "random-seeded", seed=5)[:, 0::2].reshape(-1))
+    # C2R always takes the seeded half spectrum, rows of n+2 floats
+    got = _dump(cli, tmp_path, 256, 4, AB_DIR="c2r", AB_INPUT="impulse")
+    np.testing.assert_array_equal(got, oracle.half_input(256, 4).reshape(-1))
+
+
+def test_input_file_replaces_the_generated_input(cli, tmp_path):
+    x = oracle.uniform_input(3, (2, 2 * 64))
+    f = tmp_path / "x.bin"
+    x.tofile(f)
+    got = _dump(cli, tmp_path, 64, 2, AB_INPUT_FILE=f)
+    np.testing.assert_array_equal(got, x.reshape(-1))
+
+
+def test_stale_outputs_are_flagged(cli):
+    # outputs that do not follow their inputs (the stub writes nothing)
+    for n in (1024, 8192, 65536):
+        r = run(cli, n, 2, 1, AB_INPUT_SEQ="sin,random-seeded,sin")
+        assert r.returncode == 1, r.stdout
+        rep = scopes.parse_report(r.stdout)
+        assert [s[3] for s in rep["seq"]] == [False, True, True], r.stdout
+        assert "stale-check on -> FAIL" in r.stdout
+        # identical consecutive inputs are not stale
+        r = run(cli, n, 2, 1, AB_INPUT_SEQ="zero,zero")
+        assert [s[3] for s in scopes.parse_report(r.stdout)["seq"]] == [False, False]
+
+
+def test_report_lines_follow_the_vocabulary(cli):
+    r = run(cli, 1024, 4, 3)
+    rep = scopes.parse_report(r.stdout)
+    assert rep["verdict"] == "FAIL" and rep["launches"] == 1 and rep["boundary_impl"] is None
+    f = rep["scopes"]
+    assert f["reps"] == 3 and f["h2d"] is None and f["d2h"] is None and rep["segments"] is None
+    assert scopes.infer_mode(f) == "short"
+    for n, b, impl, segs in ((8192, 300, "single", {"fft1"}), (16384, 2, "fused2", {"fft1", "fft2"}),
+                             (65536, 1, "fused2", {"fft1", "fft2"})):
+        rep = scopes.parse_report(run(cli, n, b, 2).stdout)
+        assert rep["boundary_impl"] == impl and set(rep["segments"]) == segs, (n, b)
+        assert rep["scopes"]["h2d"] is not None and rep["scopes"]["d2h"] is not None
+        assert scopes.infer_mode(rep["scopes"]) == "long_device"
+    rep = scopes.parse_report(run(cli, 8192, 3, 2, AB_VARIANT="four_step").stdout)
+    assert rep["boundary_impl"] == "fused2" and rep["launches"] == 2
+    r = run(cli, 4096, 2, 2, AB_E2E=3, AB_E2E_MODE="xfer")
+    assert "E2E n=4096 batch=2 reps=3" in r.stdout and "mode=xfer host=pinned" in r.stdout
+    assert "E2E transfers: in=1 out=1 boundary=0 per_execution" in r.stdout
+
+
+# ---------------------------------------------------------------- parser
+DEVICE = ("scopes: plan_setup=812.5 us first_use=950.0 us host_end_to_end mean=4100.0 min=3900.0 us "
+          "h2d=1500.0 us device_chain=210.0 us d2h=1450.0 us reps=5")
+SHORT = ("scopes: plan_setup=300.0 us first_use=400.0 us host_end_to_end mean=60.0 min=55.0 us "
+         "h2d=NA device_chain=47.5 us d2h=NA reps=20")
+
+
+def test_parse_scopes_fields_and_na():
+    f = scopes.parse_scopes(DEVICE)
+    assert sorted(f) == sorted(scopes.FIELDS)
+    assert (f["h2d"], f["device_chain"], f["d2h"], f["e2e_mean"], f["e2e_min"], f["reps"]) == \
+        (1500.0, 210.0, 1450.0, 4100.0, 3900.0, 5)
+    s = scopes.parse_scopes(SHORT)
+    assert s["h2d"] is None and s["d2h"] is None and s["device_chain"] == 47.5
+    for bad in (DEVICE.replace(" d2h=1450.0 us", ""), "PASS",
+                "scopes: plan_setup=1.0 us first_use=2.0 us host_end_to_end mean=3.0 min=2.0 us "
+                "device_only=1.0 us reps=1"):
+        with pytest.raises(ValueError):
+            scopes.parse_scopes(bad)
+
+
+def test_validate_scopes_rules():
+    assert scopes.validate_scopes(scopes.parse_scopes(SHORT), "short") == []
+    assert scopes.validate_scopes(scopes.parse_scopes(DEVICE), "long_device") == []
+    # the short path must not report transfers; long chains must
+    assert any("h2d=NA" in p for p in scopes.validate_scopes(scopes.parse_scopes(DEVICE), "short"))
+    assert any("h2d must be measured" in p for p in
+               scopes.validate_scopes(scopes.parse_scopes(SHORT), "long_device"))
+    # transfers folded into the chain overrun the wall clock
+    bad = scopes.parse_scopes(DEVICE.replace("device_chain=210.0", "device_chain=9000.0"))
+    assert any("transfers are not allowed" in p for p in scopes.validate_scopes(bad, "long_device"))
+    assert scopes.validate_scopes(scopes.parse_scopes(SHORT), "sideways")[0].startswith("unknown mode")
+
+
+def test_segments_and_boundary_impl():
+    assert scopes.parse_segments("segments: NA") is None
+    seg = scopes.parse_segments("segments: fft1=120.0 us fft2=90.0 us")
+    assert seg == {"fft1": 120.0, "fft2": 90.0}
+    f = scopes.parse_scopes(DEVICE)
+    assert scopes.validate_segments(seg, f, "long_device", "fused2") == []
+    assert scopes.validate_segments({"fft1": 210.0}, f, "long_device", "single") == []
+    # segments that do not add up to the chain, or belong to another chain
+    assert any("telescope" in p for p in
+               scopes.validate_segments({"fft1": 500.0, "fft2": 90.0}, f, "long_device", "fused2"))
+    assert any("missing" in p for p in scopes.validate_segments({"fft1": 210.0}, f, "long_device", "fused2"))
+    assert any("not part" in p for p in scopes.validate_segments(seg, f, "long_device", "single"))
+    assert scopes.validate_segments(None, scopes.parse_scopes(SHORT), "short") == []
+    assert scopes.validate_segments(seg, scopes.parse_scopes(SHORT), "short") != []
+    six = {k: 35.0 for k in scopes.SEGMENT_FIELDS}
+    assert scopes.validate_segments(six, f, "long_device", "separate") == []
+    with pytest.raises(ValueError):
+        scopes.parse_segments("segments: fft3=1.0 us")
+    for impl in ("single", "fused2", "fused", "separate"):
+        assert scopes.parse_boundary_impl(f"boundary_impl: {impl}") == impl
+    with pytest.raises(ValueError):
+        scopes.parse_boundary_impl("boundary_impl: bespoke")

@@ -314,3 +314,16 @@ def test_input_generators_match_reference_twin():
     for mode in oracle.NUMERICAL_MODES:
         x = oracle.numerical_input(64, 2, mode)
         assert np.all(np.isfinite(x))
+
+
+def test_sin_input_sequence():
+    # src/host/fft_check.cpp:398-399: sin(0.011 i) + 0.25 cos(0.037 i) over the
+    # flat index, rounded once to fp32; 0.25 at i = 0, and numpy's FP64
+    # evaluation agrees to the last fp32 bit (at most one ulp where the FP64
+    # value sits on a rounding boundary)
+    x = oracle.sin_input(100000)
+    assert x[0] == np.float32(0.25)
+    i = np.arange(100000, dtype=np.float64)
+    want = (np.sin(0.011 * i) + 0.25 * np.cos(0.037 * i)).astype(np.float32)
+    ulps = np.abs(x.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+    assert ulps.max() <= 1 and (ulps == 0).mean() > 0.999
