@@ -58,7 +58,7 @@ ERRORS = {
 # Every symbol include/bfly_hip.h declares (tests check the export table).
 SYMBOLS = (
     "bfly_rt_init", "bfly_rt_stream_create", "bfly_rt_stream_destroy", "bfly_rt_stream_sync",
-    "bfly_rt_malloc", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
+    "bfly_rt_malloc", "bfly_rt_mem_info", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
     "bfly_rt_memcpy_h2d_async", "bfly_rt_memcpy_d2h_async", "bfly_rt_malloc_host", "bfly_rt_free_host",
     "bfly_rt_event_create", "bfly_rt_event_destroy", "bfly_rt_event_record", "bfly_rt_event_elapsed",
     "bfly_c2c_launches", "bfly_c2c_traced", "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
@@ -94,6 +94,7 @@ def _bind(lib):
         "bfly_rt_stream_destroy": (i32, [vp]),
         "bfly_rt_stream_sync": (i32, [vp]),
         "bfly_rt_malloc": (i32, [C.POINTER(vp), sz]),
+        "bfly_rt_mem_info": (i32, [C.POINTER(sz), C.POINTER(sz)]),
         "bfly_rt_free": (i32, [vp]),
         "bfly_rt_memcpy_h2d": (i32, [vp, vp, sz, vp]),
         "bfly_rt_memcpy_d2h": (i32, [vp, vp, sz, vp]),

@@ -49,10 +49,18 @@ int bfly_rt_malloc(void** ptr, size_t bytes) {
     if (!ptr) return BFLY_ERR_SHAPE;
     *ptr = nullptr;
     if (hipMalloc(ptr, bytes ? bytes : 16) != hipSuccess) {
+        // consume the runtime's last-error record: a refused allocation must
+        // not surface as the failure of the next kernel launch
+        (void)hipGetLastError();
         *ptr = nullptr;
         return BFLY_ERR_ALLOC;
     }
     return BFLY_OK;
+}
+
+int bfly_rt_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return BFLY_ERR_SHAPE;
+    return hipMemGetInfo(free_bytes, total_bytes) == hipSuccess ? BFLY_OK : BFLY_ERR_NOT_LOADED;
 }
 
 int bfly_rt_free(void* ptr) {
@@ -90,6 +98,7 @@ int bfly_rt_malloc_host(void** ptr, size_t bytes) {
     if (!ptr) return BFLY_ERR_SHAPE;
     *ptr = nullptr;
     if (hipHostMalloc(ptr, bytes ? bytes : 16, hipHostMallocDefault) != hipSuccess) {
+        (void)hipGetLastError();
         *ptr = nullptr;
         return BFLY_ERR_ALLOC;
     }

@@ -290,6 +290,7 @@ int bfly_group_create(const bfly_group_request* reqs, int nreq, const bfly_group
     Group* G = new (std::nothrow) Group();
     if (!G) return BFLY_ERR_ALLOC;
     if (hipMalloc(&G->dev, bytes) != hipSuccess) {
+        (void)hipGetLastError();  // a refused allocation is reported here, not by a later launch
         delete G;
         return BFLY_ERR_ALLOC;
     }

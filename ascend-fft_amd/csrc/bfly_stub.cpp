@@ -64,10 +64,17 @@ int bfly_rt_stream_sync(void*) {
     ++g.syncs;
     return BFLY_OK;
 }
+// the stub's device: 16 GiB, reported free in full; larger requests fail
+constexpr size_t kStubDeviceBytes = size_t(16) << 30;
+int bfly_rt_mem_info(size_t* free_bytes, size_t* total_bytes) {
+    if (!free_bytes || !total_bytes) return BFLY_ERR_SHAPE;
+    *free_bytes = *total_bytes = kStubDeviceBytes;
+    return BFLY_OK;
+}
 int bfly_rt_malloc(void** ptr, size_t bytes) {
     if (!ptr) return BFLY_ERR_SHAPE;
     ++g.allocations;
-    if (g.allocations == g.fail_allocation) { *ptr = nullptr; return BFLY_ERR_ALLOC; }
+    if (g.allocations == g.fail_allocation || bytes > kStubDeviceBytes) { *ptr = nullptr; return BFLY_ERR_ALLOC; }
     *ptr = std::malloc(bytes ? bytes : 16);
     return *ptr ? BFLY_OK : BFLY_ERR_ALLOC;
 }

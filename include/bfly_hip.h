@@ -60,6 +60,10 @@ BFLY_API int bfly_rt_stream_create(void** stream);
 BFLY_API int bfly_rt_stream_destroy(void* stream);
 BFLY_API int bfly_rt_stream_sync(void* stream);
 BFLY_API int bfly_rt_malloc(void** ptr, size_t bytes);
+/* Free and total device memory of the current device, bytes (the HBM query
+ * the reference's stress runner makes through npu-smi,
+ * scripts/run_stress.py:151-169). */
+BFLY_API int bfly_rt_mem_info(size_t* free_bytes, size_t* total_bytes);
 BFLY_API int bfly_rt_free(void* ptr);
 BFLY_API int bfly_rt_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
 BFLY_API int bfly_rt_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);

@@ -1,5 +1,7 @@
 """Shared test setup: paths, the `gpu` marker, seeded inputs."""
 import os
+import shutil
+import subprocess
 import sys
 from pathlib import Path
 
@@ -30,3 +32,22 @@ def dev():
     if not torch.cuda.is_available():
         pytest.fail("GPU test collected without a GPU")
     return torch.device("cuda:0")
+
+
+@pytest.fixture(scope="session")
+def stub_cli(tmp_path_factory):
+    """The bfly_check program built against the C ABI's CPU stub (ASan +
+    UBSan): every guard runs, no transform is computed."""
+    if not shutil.which("g++"):
+        pytest.skip("needs g++")
+    stub = PKG / "lib" / "libbfly_stub.so"
+    assert stub.exists(), "build first: make -C ascend-fft_amd"
+    exe = tmp_path_factory.mktemp("cli") / "bfly_check_stub"
+    subprocess.run(
+        ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-fsanitize=address,undefined",
+         f"-I{ROOT / 'include'}", f"-I{PKG / 'csrc'}", f"-I{PKG / 'csrc' / 'check'}",
+         str(PKG / "csrc" / "check" / "bfly_check.cpp"), str(PKG / "csrc" / "check" / "inputs.cpp"),
+         str(PKG / "csrc" / "framework" / "reference.cpp"),
+         f"-L{stub.parent}", "-lbfly_stub", f"-Wl,-rpath,{stub.parent}", "-o", str(exe)],
+        check=True, capture_output=True, text=True)
+    return exe

@@ -9,7 +9,6 @@ reference's fft_check generators bit for bit.  Argument handling follows
 src/host/fft_check.cpp:213-300 (exit 2 on a bad argument).
 """
 import os
-import shutil
 import subprocess
 from pathlib import Path
 
@@ -24,20 +23,8 @@ PKG = ROOT / "ascend-fft_amd"
 
 
 @pytest.fixture(scope="module")
-def cli(tmp_path_factory):
-    if not shutil.which("g++"):
-        pytest.skip("needs g++")
-    stub = PKG / "lib" / "libbfly_stub.so"
-    assert stub.exists(), "build first: make -C ascend-fft_amd"
-    exe = tmp_path_factory.mktemp("cli") / "bfly_check_stub"
-    subprocess.run(
-        ["g++", "-std=c++17", "-O1", "-Wall", "-Wextra", "-fsanitize=address,undefined",
-         f"-I{ROOT / 'include'}", f"-I{PKG / 'csrc'}", f"-I{PKG / 'csrc' / 'check'}",
-         str(PKG / "csrc" / "check" / "bfly_check.cpp"), str(PKG / "csrc" / "check" / "inputs.cpp"),
-         str(PKG / "csrc" / "framework" / "reference.cpp"),
-         f"-L{stub.parent}", "-lbfly_stub", f"-Wl,-rpath,{stub.parent}", "-o", str(exe)],
-        check=True, capture_output=True, text=True)
-    return exe
+def cli(stub_cli):
+    return stub_cli
 
 
 def run(exe, *args, **envs):
