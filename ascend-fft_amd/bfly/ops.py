@@ -88,6 +88,52 @@ def c2c(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
     return out
 
 
+def table2d_cols(n1: int, device) -> torch.Tensor:
+    """Device copy of the 2-D column table of n1 (bfly_fft2d_table_fill), cached."""
+    device = torch.device(device)
+    key = (device.index, "fft2d_cols", n1)
+    t = _tables.get(key)
+    if t is None:
+        lib = _lib.hip()
+        nf = lib.bfly_fft2d_table_floats(n1)
+        if nf == 0:
+            raise _lib.BflyError(_lib.ERR_SHAPE, f"no 2-D column kernel for n1={n1}")
+        host = np.empty(nf, np.float32)
+        _lib.check(lib.bfly_fft2d_table_fill(n1, host.ctypes.data), "bfly_fft2d_table_fill")
+        t = torch.from_numpy(host).to(device)
+        _tables[key] = t
+    return t
+
+
+def fft2_raw(x: torch.Tensor, out: torch.Tensor, n1: int, n2: int, batch: int, sign: int = -1,
+             work: torch.Tensor | None = None) -> None:
+    """bfly_fft2d on interleaved float32 buffers [batch][n1][2 n2]."""
+    lib = _lib.hip()
+    if work is None:
+        work = torch.empty(max(1, lib.bfly_fft2d_workspace_bytes(n1, n2, batch) // 4), dtype=torch.float32,
+                           device=x.device)
+    rc = lib.bfly_fft2d(x.data_ptr(), out.data_ptr(), n1, n2, batch, sign,
+                        table(_lib.KIND_C2C, n2, x.device).data_ptr(), table2d_cols(n1, x.device).data_ptr(),
+                        work.data_ptr(), _stream(x.device))
+    _lib.check(rc, f"bfly_fft2d(n1={n1}, n2={n2}, batch={batch}, sign={sign})")
+
+
+def fft2(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
+    """Unnormalised 2-D C2C over the last two axes of a complex64 tensor
+    [..., n1, n2] (sign -1 forward, +1 inverse) = torch.fft.fft2 (norm
+    "backward") for sign -1."""
+    _check_device(x, "x")
+    if x.dtype != torch.complex64 or x.dim() < 2:
+        raise TypeError("fft2 takes a complex64 tensor [..., n1, n2]")
+    n1, n2 = x.shape[-2], x.shape[-1]
+    batch = x.numel() // (n1 * n2) if n1 * n2 else 0
+    out = torch.empty_like(x)
+    if batch == 0:
+        return out
+    fft2_raw(torch.view_as_real(x), torch.view_as_real(out), n1, n2, batch, sign)
+    return out
+
+
 def _real(kind: int, src: torch.Tensor, dst: torch.Tensor, n: int, batch: int, bins: int, stride: int,
           adjoint: bool, what: str) -> None:
     tab = table(kind, n, src.device)

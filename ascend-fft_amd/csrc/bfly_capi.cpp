@@ -136,6 +136,20 @@ int bfly_rt_event_elapsed(void* start, void* end, float* ms) {
     return hipEventElapsedTime(ms, a, b) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
 }
 
+size_t bfly_fft2d_table_floats(uint32_t n1) { return 2 * bfly_host::fft2d_layout(n1).complex_len; }
+
+int bfly_fft2d_table_fill(uint32_t n1, float* host_table) { return bfly_host::fft2d_fill(n1, host_table); }
+
+size_t bfly_fft2d_workspace_bytes(uint32_t n1, uint32_t n2, uint32_t batch) {
+    return bfly_host::fft2d_ok(n1, n2) ? 8 * (size_t)n1 * n2 * batch : 0;
+}
+
+int bfly_fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign,
+               const float* table_rows, const float* table_cols, void* work, void* stream) {
+    if (int g = bfly_host::check_fft2d(sign, in, out, n1, n2, batch, table_rows, table_cols, work)) return g;
+    return launched(bfly_kern::fft2d(in, out, n1, n2, batch, sign, table_rows, table_cols, work, S(stream)));
+}
+
 int bfly_c2c_launches(uint32_t n, uint32_t batch, int variant) {
     if (!bfly_supported(BFLY_KIND_C2C, n) || batch == 0) return BFLY_ERR_SHAPE;
     if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;

@@ -1,0 +1,167 @@
+// bfly_fft2d.hip -- 2-D C2C transforms of [batch][n1][n2] interleaved
+// complex arrays (row-major: n1 rows of n2 points).
+//
+// X[k1][k2] = sum_{j1, j2} x[j1][j2] W_n1^{j1 k1} W_n2^{j2 k2} is separable:
+// the row pass is the C2C kernels over batch * n1 rows of n2 points; the
+// column pass runs k_cols, which takes C = 256 / T adjacent columns per
+// workgroup (T = threads per column transform), so every load and store is
+// a run of C consecutive complex values (C * 8 bytes: 128 B at 256-point
+// columns, 512 B at 64).  Column lengths up to 256 are one k_cols pass;
+// 512..4096 are split n1 = P * Q (four-step along the columns):
+//
+//   j1 = Q p + q,  k1 = kp + P kq
+//   Y[Q kp + q][c] = W_n1^{q kp} * FFT_P over p of x[Q p + q][c]   (k_cols, twiddle)
+//   X[kp + P kq][c] = FFT_Q over q of Y[Q kp + q][c]              (k_cols, row map)
+//
+// so the whole transform is two HBM round trips up to n1 = 256 and three
+// above.  The column transforms use the LDS twiddle tables of the streaming
+// C2C kernels (stockham_lt); the inter-pass twiddle is W_n1^e =
+// A[e mod 64] * B[e / 64] from two LDS tables.
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "bfly_hip.h"
+#include "bfly_kernels.hpp"
+#include "bfly_tables.hpp"
+#include "bfly_units.hpp"
+
+namespace {
+
+using namespace bfly_dev;
+
+// Addressing of one column pass.  A batch item of `item` elements holds
+// `mats` matrices; matrix m of item b reads rows at in + b*item +
+// m*in_mstride (row pitch in_rstride) and writes at out + b*item +
+// m*out_mstride (row pitch out_rstride); columns are `ncols` wide, walked
+// by `groups` workgroups of C columns.
+struct ColView {
+    uint64_t item;
+    uint64_t in_mstride, out_mstride;
+    uint32_t in_rstride, out_rstride;
+    uint32_t ncols, groups, mats;
+    uint32_t col_shift;  // twiddle pass: q = column >> col_shift
+};
+
+template <int I, int DIR, bool TW, int WG = bfly_plan::kBlock>
+__global__ __launch_bounds__(WG) void k_cols(const float2* __restrict__ in, float2* __restrict__ out,
+                                              const float2* __restrict__ tw, const float2* __restrict__ wab,
+                                              ColView cv) {
+    using PL = PlanOf<I>;
+    using LT = C2cLdsTw<I>;
+    constexpr int L = PL::N, T = PL::T, C = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int STRIDE = lds_len(L) + 1;
+    // the twiddle is read after the transform, whose LDS exchange (two or
+    // more passes) orders the table writes before it
+    static_assert(!TW || PL::NRAD > 1, "twiddle pass needs a multi-pass column plan");
+    __shared__ float2 lds[C * STRIDE];
+    __shared__ float2 ltw[LT::LEN];
+    __shared__ float2 wt[TW ? 128 : 1];
+    const int tid = threadIdx.x;
+    // the column plan's tables: [its pass tables | A | B] as in the local C2C table
+    c2c_lds_tw<I, WG>(ltw, tw, tw + bfly_plan::stage_twiddle_len(I), tid);
+    if constexpr (TW)
+        for (int i = tid; i < 128; i += WG) wt[i] = wab[i];
+    const int c = tid % C, t = tid / C;
+    const uint32_t blk = blockIdx.x;
+    const uint32_t g = blk % cv.groups;
+    const uint64_t mi = blk / cv.groups;
+    const uint64_t b = mi / cv.mats, m = mi % cv.mats;
+    const uint32_t col = g * C + c;
+    const bool active = col < cv.ncols;
+    const float2* src = in + b * cv.item + m * cv.in_mstride + col;
+    float2 v[P];
+#pragma unroll
+    for (int mm = 0; mm < P / R0; mm++)
+#pragma unroll
+        for (int r = 0; r < R0; r++)
+            v[mm * R0 + r] = active ? ld_stream(&src[(uint64_t)cv.in_rstride * (t + mm * T + r * (L / R0))])
+                                    : make_float2(0.f, 0.f);
+    // the transform's first LDS exchange orders the table writes before any read
+    run_plan_lt<I, T, DIR, LX_PAD>(v, lds + c * STRIDE, 0, ltw, ltw + LT::NLO, ltw + LT::NLO + LT::NHI, t,
+                                   std::make_index_sequence<PL::NRAD>{});
+    if (!active) return;
+    float2* dst = out + b * cv.item + m * cv.out_mstride + col;
+    const uint32_t q = TW ? col >> cv.col_shift : 0u;
+#pragma unroll
+    for (int mm = 0; mm < P / RL; mm++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const int k = t + mm * T + r * (L / RL);
+            float2 x = v[mm * RL + r];
+            if constexpr (TW) {
+                const uint32_t e = q * (uint32_t)k;  // < n1 <= 4096
+                const float2 a = wt[e & 63], w = wt[64 + (e >> 6)];
+                x = cmul_tw<DIR>(x, make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x));
+            }
+            st_stream(&dst[(uint64_t)cv.out_rstride * k], x);
+        }
+}
+
+template <int I, bool TW>
+hipError_t launch_cols(const float2* in, float2* out, const float2* tw, const float2* wab, ColView cv,
+                       uint64_t batch, int sign, hipStream_t s) {
+    constexpr int C = bfly_plan::kBlock / PlanOf<I>::T;
+    cv.groups = (cv.ncols + C - 1) / C;
+    const uint64_t blocks = batch * cv.mats * cv.groups;
+    if (blocks == 0 || blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    if (sign < 0)
+        hipLaunchKernelGGL((k_cols<I, -1, TW>), dim3((unsigned)blocks), dim3(bfly_plan::kBlock), 0, s, in, out, tw,
+                           wab, cv);
+    else
+        hipLaunchKernelGGL((k_cols<I, 1, TW>), dim3((unsigned)blocks), dim3(bfly_plan::kBlock), 0, s, in, out, tw,
+                           wab, cv);
+    return hipGetLastError();
+}
+
+template <bool TW>
+hipError_t cols(uint32_t len, const float2* in, float2* out, const float2* tw, const float2* wab, ColView cv,
+                uint64_t batch, int sign, hipStream_t s) {
+    constexpr auto idx = bfly_plan::plan_index;
+    // the twiddle pass runs the first factor of a split length (32 or 64)
+    if constexpr (!TW)
+        if (len == 16) return launch_cols<idx(16), TW>(in, out, tw, wab, cv, batch, sign, s);
+    switch (len) {
+        case 32: return launch_cols<idx(32), TW>(in, out, tw, wab, cv, batch, sign, s);
+        case 64: return launch_cols<idx(64), TW>(in, out, tw, wab, cv, batch, sign, s);
+        case 128: return launch_cols<idx(128), TW>(in, out, tw, wab, cv, batch, sign, s);
+        case 256: return launch_cols<idx(256), TW>(in, out, tw, wab, cv, batch, sign, s);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+namespace bfly_kern {
+
+hipError_t fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign,
+                 const float* table_rows, const float* table_cols, void* work, hipStream_t s) {
+    const float2* i2 = reinterpret_cast<const float2*>(in);
+    float2* o2 = reinterpret_cast<float2*>(out);
+    float2* ws = reinterpret_cast<float2*>(work);
+    const float2* tc = reinterpret_cast<const float2*>(table_cols);
+    const bfly_host::Fft2dLayout L = bfly_host::fft2d_layout(n1);
+    const uint64_t item = (uint64_t)n1 * n2;
+    if (L.q == 1) {
+        // rows in -> ws, columns ws -> out
+        hipError_t e = c2c(in, reinterpret_cast<float*>(ws), n2, batch * n1, sign, BFLY_VARIANT_AUTO, table_rows,
+                           nullptr, s);
+        if (e != hipSuccess) return e;
+        ColView cv{item, 0, 0, n2, n2, n2, 0, 1, 0};
+        return cols<false>(n1, ws, o2, tc + L.p_table, nullptr, cv, batch, sign, s);
+    }
+    // rows in -> out, columns (P-point, twiddle) out -> ws, (Q-point, row map) ws -> out
+    hipError_t e = c2c(in, out, n2, batch * n1, sign, BFLY_VARIANT_AUTO, table_rows, nullptr, s);
+    if (e != hipSuccess) return e;
+    uint32_t shift = 0;
+    while ((1u << shift) < n2) shift++;
+    const uint32_t qn2 = L.q * n2;
+    ColView a{item, 0, 0, qn2, qn2, qn2, 0, 1, shift};
+    e = cols<true>(L.p, o2, ws, tc + L.p_table, tc + L.wab, a, batch, sign, s);
+    if (e != hipSuccess) return e;
+    ColView b{item, qn2, n2, n2, L.p * n2, n2, 0, L.p, 0};
+    return cols<false>(L.q, ws, o2, tc + L.q_table, nullptr, b, batch, sign, s);
+}
+
+}  // namespace bfly_kern

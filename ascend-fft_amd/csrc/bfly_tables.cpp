@@ -88,6 +88,56 @@ TableLayout table_layout(int kind, uint32_t n) {
     return t;
 }
 
+bool fft2d_ok(uint32_t n1, uint32_t n2) {
+    return pow2(n1) && pow2(n2) && n1 >= 16u && n2 >= 16u && n1 <= (uint32_t)bfly_plan::kMaxLocalN &&
+           n2 <= (uint32_t)bfly_plan::kMaxLocalN;
+}
+
+Fft2dLayout fft2d_layout(uint32_t n1) {
+    Fft2dLayout t{};
+    if (!pow2(n1) || n1 < 16u || n1 > (uint32_t)bfly_plan::kMaxLocalN) return t;
+    if (n1 <= 256u) {
+        t.p = n1;
+        t.q = 1;
+    } else {
+        t.p = n1 >= 2048u ? 64u : 32u;
+        t.q = n1 / t.p;
+    }
+    t.p_table = 0;
+    t.q_table = table_layout(BFLY_KIND_C2C, t.p).complex_len;
+    t.wab = t.q_table + (t.q > 1 ? table_layout(BFLY_KIND_C2C, t.q).complex_len : 0);
+    t.complex_len = t.wab + (t.q > 1 ? 128 : 0);
+    return t;
+}
+
+int fft2d_fill(uint32_t n1, float* dst) {
+    const Fft2dLayout t = fft2d_layout(n1);
+    if (!dst || !t.complex_len) return BFLY_ERR_SHAPE;
+    for (size_t i = 0; i < 2 * t.complex_len; i++) dst[i] = 0.f;
+    int rc = fill_table(BFLY_KIND_C2C, t.p, dst + 2 * t.p_table);
+    if (rc == BFLY_OK && t.q > 1) rc = fill_table(BFLY_KIND_C2C, t.q, dst + 2 * t.q_table);
+    if (rc != BFLY_OK || t.q == 1) return rc;
+    for (uint64_t j = 0; j < 64; j++) twiddle(j, n1, dst + 2 * (t.wab + j));
+    for (uint64_t j = 0; j < n1 / 64; j++) twiddle(64 * j, n1, dst + 2 * (t.wab + 64 + j));
+    return BFLY_OK;
+}
+
+int check_fft2d(int sign, const void* in, const void* out, uint32_t n1, uint32_t n2, uint32_t batch,
+                const void* table_rows, const void* table_cols, const void* work) {
+    if (!fft2d_ok(n1, n2) || batch == 0 || !in || !out || !work) return BFLY_ERR_SHAPE;
+    if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
+    if (!table_rows || !table_cols) return BFLY_ERR_TABLE;
+    const uint64_t item = (uint64_t)n1 * n2;
+    // row pass: batch * n1 transforms in one launch; column pass grids
+    if ((uint64_t)batch * n1 > 0xffffffffull || (uint64_t)batch * item / 16 > 0x7fffffffull)
+        return BFLY_ERR_SHAPE;
+    const uint64_t bytes = 8 * item * batch;
+    const auto lo = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+    const auto apart = [&](const void* a, const void* b) { return lo(a) + bytes <= lo(b) || lo(b) + bytes <= lo(a); };
+    if (!apart(in, out) || !apart(in, work) || !apart(out, work)) return BFLY_ERR_SHAPE;
+    return BFLY_OK;
+}
+
 int fill_table(int kind, uint32_t n, float* dst) {
     if (!dst || !shape_ok(kind, n)) return BFLY_ERR_SHAPE;
     const TableLayout t = table_layout(kind, n);

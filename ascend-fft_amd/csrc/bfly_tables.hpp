@@ -49,6 +49,22 @@ bool needs_work(int kind, uint32_t n, uint32_t batch, int variant);
 // Lengths the split-FP16 matrix-core variant runs (plans [16, 16, R]).
 bool split_mfma_ok(uint32_t n);
 
+// 2-D C2C of [batch][n1][n2]: n1, n2 powers of two in [16, 4096].  Column
+// lengths up to 256 run as one pass (p = n1, q = 1); above, n1 = p * q
+// (p = 32 / 32 / 64 / 64 for 512 / 1024 / 2048 / 4096).  The column table
+// of n1 is [local C2C table of p][local C2C table of q][W_n1^j, j < 64 |
+// W_n1^{64 j}, j < n1 / 64, zero-padded to 128 entries] (complex offsets).
+struct Fft2dLayout {
+    uint32_t p = 0, q = 0;
+    size_t p_table = 0, q_table = 0, wab = 0, complex_len = 0;
+};
+bool fft2d_ok(uint32_t n1, uint32_t n2);
+Fft2dLayout fft2d_layout(uint32_t n1);
+int fft2d_fill(uint32_t n1, float* dst);
+// Guards of bfly_fft2d: BFLY_OK or the status code.
+int check_fft2d(int sign, const void* in, const void* out, uint32_t n1, uint32_t n2, uint32_t batch,
+                const void* table_rows, const void* table_cols, const void* work);
+
 // Mode-truncated real transforms: 1 <= bins <= n/2+1, stride (floats) even
 // and at least 2*bins.
 bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride);

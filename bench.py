@@ -377,6 +377,51 @@ def grid49_pattern(reps: int = 50):
     }
 
 
+def fft2d_pattern(reps: int = 30):
+    """2-D C2C fp32 on the reference's declared 2-D shapes (fft2d squares and
+    range-Doppler, config/test_matrix.json:94-99), batch sized to 256 MiB per
+    tensor: the library (bfly_fft2d) against torch.fft.fft2 (hipFFT) on the
+    same device tensors, HIP events; algorithmic bytes 16 n1 n2 batch."""
+    import torch
+    from bfly import ops
+
+    dev = torch.device("cuda:0")
+    g = torch.Generator(device=dev)
+    g.manual_seed(6)
+
+    def timed(fn):
+        for _ in range(3):
+            fn()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(reps):
+            fn()
+        e1.record()
+        e1.synchronize()
+        return e0.elapsed_time(e1) / reps * 1e3
+
+    points = []
+    for n1, n2 in ((256, 256), (1024, 1024), (2048, 2048), (1024, 128), (2048, 256), (4096, 256)):
+        b = max(1, (32 << 20) // (n1 * n2))
+        x = torch.complex(torch.rand(b, n1, n2, generator=g, device=dev) * 2 - 1,
+                          torch.rand(b, n1, n2, generator=g, device=dev) * 2 - 1)
+        y = torch.empty_like(x)
+        work = torch.empty(2 * x.numel(), dtype=torch.float32, device=dev)
+        xf, yf = torch.view_as_real(x), torch.view_as_real(y)
+        ours = timed(lambda: ops.fft2_raw(xf, yf, n1, n2, b, -1, work))
+        ref = timed(lambda: torch.fft.fft2(x))
+        want = torch.fft.fft2(x)
+        err = float(torch.linalg.vector_norm(y - want) / torch.linalg.vector_norm(want))
+        nbytes = 16 * n1 * n2 * b
+        points.append({"n1": n1, "n2": n2, "batch": b, "us": round(ours, 2), "GB_per_s": round(nbytes / ours / 1e3, 1),
+                       "hbm_round_trips": 2 if n1 <= 256 else 3, "us_torch_fft2": round(ref, 2),
+                       "speedup": round(ref / ours, 3), "rel_l2_vs_torch": err})
+        del x, y, work
+    return {"workload": "2-D C2C fp32 [batch][n1][n2], 256 MiB per tensor, seed 6, vs torch.fft.fft2 (hipFFT)",
+            "points": points}
+
+
 def workload_loop(name: str, steps: int, warmup: int):
     """One secondary workload run `steps` times after `warmup` untimed runs,
     for kernel traces and counter passes of its kernels (--only NAME).
@@ -558,7 +603,7 @@ def main():
     if not args.no_patterns:
         line["rows"] = secondary_rows(0 if args.no_cpu else 1.5)
         line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern(),
-                            "grid49_vs_torch_fft": grid49_pattern()}
+                            "grid49_vs_torch_fft": grid49_pattern(), "fft2d_vs_torch_fft2": fft2d_pattern()}
     print(json.dumps(line))
 
 

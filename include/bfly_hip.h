@@ -125,6 +125,22 @@ BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
 BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream);
 
+/* ------------------------------------------------------------------ 2-D
+ * 2-D C2C of `batch` row-major [n1][n2] arrays of interleaved complex
+ * values: X[k1][k2] = sum x[j1][j2] e^{sign 2 pi i (j1 k1 / n1 + j2 k2 / n2)}
+ * (sign -1 forward, +1 unnormalised inverse); n1, n2 powers of two in
+ * [16, 4096].  No counterpart in the reference, which lists rectangular 2-D
+ * transforms (fft2d, range-Doppler) as future work
+ * (config/test_matrix.json:94-99).  `table_rows` is the device copy of
+ * bfly_table_fill(BFLY_KIND_C2C, n2), `table_cols` of bfly_fft2d_table_fill(n1);
+ * `work` holds bfly_fft2d_workspace_bytes(n1, n2, batch) bytes.  Two HBM
+ * round trips up to n1 = 256, three above. */
+BFLY_API size_t bfly_fft2d_table_floats(uint32_t n1);
+BFLY_API int bfly_fft2d_table_fill(uint32_t n1, float* host_table);
+BFLY_API size_t bfly_fft2d_workspace_bytes(uint32_t n1, uint32_t n2, uint32_t batch);
+BFLY_API int bfly_fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign,
+                        const float* table_rows, const float* table_cols, void* work, void* stream);
+
 /* Kernel launches bfly_c2c_ex makes for (n, batch, variant): 1, or 2 for
  * the four-step chain (the column kernel -- transpose in, N1-point FFTs,
  * twiddle -- then the row kernel -- N2-point FFTs, natural-order store);

@@ -52,6 +52,7 @@ def _bind(lib):
     u32, i32, sz = C.c_uint32, C.c_int, C.c_size_t
     fn("orc_fft_f64_inplace", None, _f64p, u32, i32)
     fn("orc_c2c_f64", None, _f32p, _f64p, u32, u32, i32)
+    fn("orc_fft2_f64", None, _f32p, _f64p, u32, u32, u32, i32)
     fn("orc_ref_fft_f32", None, _f32p, _f32p, u32)
     fn("orc_dft_direct_f64", None, _f32p, _f64p, u32, i32)
     fn("orc_ref_r2c_f32", None, _f32p, _f32p, u32)
@@ -162,6 +163,14 @@ def c2c_f64(x: np.ndarray, n: int, sign: int = -1) -> np.ndarray:
     x = _f32(x).reshape(-1, 2 * n)
     out = np.empty(x.shape, np.float64)
     lib().orc_c2c_f64(x, out, n, x.shape[0], sign)
+    return out
+
+
+def fft2_f64(x: np.ndarray, n1: int, n2: int, sign: int = -1) -> np.ndarray:
+    """FP64 oracle 2-D C2C of interleaved fp32 arrays [batch][n1][2 n2]."""
+    x = _f32(x).reshape(-1, n1 * 2 * n2)
+    out = np.empty(x.shape, np.float64)
+    lib().orc_fft2_f64(x, out, n1, n2, x.shape[0], sign)
     return out
 
 

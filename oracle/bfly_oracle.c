@@ -91,6 +91,30 @@ ORC_API void orc_c2c_f64(const float* in, double* out, uint32_t n, uint32_t batc
     }
 }
 
+/* 2-D C2C of `batch` row-major [n1][n2] interleaved fp32 arrays in FP64:
+ * the 1-D FP64 FFT along every row, then along every column.  No reference
+ * counterpart (2-D transforms are future work there,
+ * config/test_matrix.json:94-99); the definition is the separable 2-D DFT. */
+ORC_API void orc_fft2_f64(const float* in, double* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign) {
+    orc_c2c_f64(in, out, n2, batch * n1, sign);
+    double* col = (double*)malloc(sizeof(double) * 2 * n1);
+    for (uint32_t b = 0; b < batch; b++) {
+        double* m = out + (size_t)b * 2 * n1 * n2;
+        for (uint32_t c = 0; c < n2; c++) {
+            for (uint32_t r = 0; r < n1; r++) {
+                col[2 * r] = m[2 * ((size_t)r * n2 + c)];
+                col[2 * r + 1] = m[2 * ((size_t)r * n2 + c) + 1];
+            }
+            orc_fft_f64_inplace(col, n1, sign);
+            for (uint32_t r = 0; r < n1; r++) {
+                m[2 * ((size_t)r * n2 + c)] = col[2 * r];
+                m[2 * ((size_t)r * n2 + c) + 1] = col[2 * r + 1];
+            }
+        }
+    }
+    free(col);
+}
+
 /* refFftF32 (src/framework/reference.cpp:14-50): FP64 compute, fp32 result. */
 ORC_API void orc_ref_fft_f32(const float* in, float* out, uint32_t n) {
     if (!n) return;

@@ -214,6 +214,54 @@ def test_chain_launch_count_and_traced_guards_through_stub():
     assert cnt[3] == 2
 
 
+@pytest.mark.parametrize("n1", [16, 64, 256, 512, 1024, 2048, 4096])
+def test_fft2d_column_table_layout(n1):
+    s = bfly.stub()
+    nf = s.bfly_fft2d_table_floats(n1)
+    tab = np.empty(nf, np.float32)
+    assert s.bfly_fft2d_table_fill(n1, tab.ctypes.data) == 0
+    c = tab.view(np.complex64)
+    p = n1 if n1 <= 256 else (64 if n1 >= 2048 else 32)
+    q = n1 // p
+    # [local C2C table of p][local C2C table of q][W_n1^j, j<64 | W_n1^{64 j}] (128 entries)
+    lp = s.bfly_table_floats(bfly.KIND_C2C, p) // 2
+    tp = np.empty(2 * lp, np.float32)
+    assert s.bfly_table_fill(bfly.KIND_C2C, p, tp.ctypes.data) == 0
+    np.testing.assert_array_equal(c[:lp], tp.view(np.complex64))
+    if q == 1:
+        assert c.size == lp
+        return
+    lq = s.bfly_table_floats(bfly.KIND_C2C, q) // 2
+    tq = np.empty(2 * lq, np.float32)
+    assert s.bfly_table_fill(bfly.KIND_C2C, q, tq.ctypes.data) == 0
+    np.testing.assert_array_equal(c[lp:lp + lq], tq.view(np.complex64))
+    w = c[lp + lq:]
+    assert w.size == 128
+    j = np.arange(64)
+    np.testing.assert_allclose(w[:64], np.exp(-2j * np.pi * j / n1), atol=1e-7)
+    np.testing.assert_allclose(w[64:64 + n1 // 64], np.exp(-2j * np.pi * 64 * np.arange(n1 // 64) / n1), atol=1e-7)
+    assert not w[64 + n1 // 64:].any()
+
+
+def test_fft2d_guards_through_stub():
+    s = bfly.stub()
+    buf = np.zeros(16, np.float32)
+    p = buf.ctypes.data
+    i, o, w, t, tc = p, p + (1 << 30), p + (2 << 30), p + (3 << 30), p + (4 << 30)
+    assert s.bfly_fft2d(i, o, 64, 128, 3, -1, t, tc, w, None) == 0
+    assert s.bfly_fft2d(i, o, 4096, 4096, 1, 1, t, tc, w, None) == 0
+    assert s.bfly_fft2d_workspace_bytes(64, 128, 3) == 8 * 64 * 128 * 3
+    assert s.bfly_fft2d_workspace_bytes(8, 128, 3) == 0 and s.bfly_fft2d_table_floats(8192) == 0
+    for n1, n2 in ((8, 64), (64, 8192), (48, 64), (64, 0)):
+        assert s.bfly_fft2d(i, o, n1, n2, 1, -1, t, tc, w, None) == bfly.ERR_SHAPE
+    assert s.bfly_fft2d(i, o, 64, 64, 0, -1, t, tc, w, None) == bfly.ERR_SHAPE
+    assert s.bfly_fft2d(i, o, 64, 64, 1, -1, t, tc, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_fft2d(i, o, 64, 64, 1, -1, None, tc, w, None) == _lib.ERR_TABLE
+    assert s.bfly_fft2d(i, i + 8 * 64 * 64 - 8, 64, 64, 1, -1, t, tc, w, None) == bfly.ERR_SHAPE
+    assert s.bfly_fft2d(i, o, 64, 64, 1, -1, t, tc, o + 8, None) == bfly.ERR_SHAPE
+    assert s.bfly_fft2d(i, i + 8 * 64 * 64, 64, 64, 1, -1, t, tc, w, None) == 0  # adjacent
+
+
 def test_python_modules_import_and_variant_ids_match_header():
     import importlib
     for m in ("bfly.ops", "bfly.tune", "bfly.autograd", "bfly.spectral"):

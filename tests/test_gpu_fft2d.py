@@ -1,0 +1,88 @@
+"""GPU: the 2-D C2C transform (bfly_fft2d; bfly.ops.fft2) against the FP64
+oracle (relative L2 <= 1e-5 per 2-D transform) and torch.fft.fft2, for the
+one-pass column lengths (n1 <= 256), the split ones (512..4096), both signs,
+ragged batches, and the reference's declared 2-D shapes (fft2d 256..2048
+squares, range-Doppler 1024x128 / 2048x256 / 4096x256)."""
+import numpy as np
+import pytest
+
+import oracle
+from conftest import gpu_available
+
+pytestmark = [pytest.mark.gpu,
+              pytest.mark.skipif(not gpu_available(), reason="needs an MI355X")]
+
+REL_L2 = 1e-5
+
+SHAPES = [
+    (16, 16, 3), (16, 4096, 2), (32, 64, 5), (64, 128, 4), (128, 16, 7), (256, 256, 4), (256, 4096, 1),
+    (512, 512, 2), (512, 16, 3), (1024, 128, 4), (1024, 1024, 1), (2048, 256, 2), (2048, 2048, 1),
+    (4096, 256, 1), (4096, 16, 2), (16, 1024, 9),
+]
+
+
+def _run(dev, x, sign):
+    import torch
+    from bfly import ops
+    xt = torch.from_numpy(x).to(dev)
+    b, n1, n2 = x.shape
+    y = ops.fft2(xt, sign)
+    torch.cuda.synchronize()
+    return y.cpu().numpy()
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+@pytest.mark.parametrize("n1,n2,batch", SHAPES, ids=[f"{a}x{b}-b{c}" for a, b, c in SHAPES])
+def test_fft2_matches_the_oracle(dev, n1, n2, batch, sign):
+    print(f"fft2 n1={n1} n2={n2} batch={batch} sign={sign}", flush=True)
+    rng = np.random.default_rng(n1 * 7919 + n2 + batch)
+    z = rng.uniform(-1, 1, (batch, n1, n2, 2)).astype(np.float32)
+    x = z[..., 0] + 1j * z[..., 1]
+    y = _run(dev, x.astype(np.complex64), sign)
+    want = oracle.fft2_f64(z.reshape(batch, n1, 2 * n2), n1, n2, sign)
+    got = np.ascontiguousarray(y).view(np.float32).reshape(batch, -1)
+    assert oracle.rel_l2_rows(got, want) <= REL_L2
+
+
+def test_fft2_matches_torch_and_special_inputs(dev):
+    import torch
+    from bfly import ops
+    g = torch.Generator(device="cpu").manual_seed(4)
+    for n1, n2 in ((1024, 128), (256, 256), (4096, 16)):
+        x = torch.complex(torch.rand(3, n1, n2, generator=g) * 2 - 1,
+                          torch.rand(3, n1, n2, generator=g) * 2 - 1).to(dev)
+        y = ops.fft2(x)
+        ref = torch.fft.fft2(x)
+        rel = (torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref)).item()
+        assert rel < 3e-6, (n1, n2, rel)
+    # impulse at (a, c): e^{-2 pi i (a k1 / n1 + c k2 / n2)}; constant: n1 n2 at the origin
+    n1, n2, a, c = 2048, 64, 77, 5
+    x = torch.zeros(1, n1, n2, dtype=torch.complex64, device=dev)
+    x[0, a, c] = 1
+    y = ops.fft2(x)[0].cpu().numpy()
+    k1, k2 = np.meshgrid(np.arange(n1), np.arange(n2), indexing="ij")
+    assert np.abs(y - np.exp(-2j * np.pi * (a * k1 / n1 + c * k2 / n2))).max() < 2e-6
+    y = ops.fft2(torch.ones(2, 512, 32, dtype=torch.complex64, device=dev)).cpu().numpy()
+    assert y[:, 0, 0].real.tolist() == [512 * 32] * 2 and np.abs(y.reshape(2, -1)[:, 1:]).max() < 1e-2
+
+
+def test_fft2_round_trip_and_guards(dev):
+    import ctypes as C
+
+    import torch
+    from bfly import _lib, ops
+    x = torch.complex(torch.randn(2, 512, 256), torch.randn(2, 512, 256)).to(dev)
+    back = ops.fft2(ops.fft2(x), sign=1) / (512 * 256)
+    assert (torch.linalg.vector_norm(back - x) / torch.linalg.vector_norm(x)).item() < 2e-6
+    lib = _lib.hip()
+    buf = torch.empty(1 << 20, dtype=torch.float32, device=dev)
+    p = buf.data_ptr()
+    t = ops.table(_lib.KIND_C2C, 64, dev).data_ptr()
+    tc = ops.table2d_cols(64, dev).data_ptr()
+    s = torch.cuda.current_stream(dev).cuda_stream
+    # shapes outside [16, 4096] or not powers of two, overlapping buffers, bad sign
+    for n1, n2 in ((8, 64), (64, 8192), (96, 64)):
+        assert lib.bfly_fft2d(p, p + (1 << 19), n1, n2, 1, -1, t, tc, p + (2 << 19), s) == _lib.ERR_SHAPE
+    assert lib.bfly_fft2d(p, p + 8, 64, 64, 1, -1, t, tc, p + (2 << 19), s) == _lib.ERR_SHAPE
+    assert lib.bfly_fft2d(p, p + (1 << 19), 64, 64, 1, 0, t, tc, p + (2 << 19), s) == _lib.ERR_SHAPE
+    assert lib.bfly_fft2d(p, p + (1 << 19), 64, 64, 1, -1, None, tc, p + (2 << 19), s) == _lib.ERR_TABLE

@@ -327,3 +327,33 @@ def test_sin_input_sequence():
     want = (np.sin(0.011 * i) + 0.25 * np.cos(0.037 * i)).astype(np.float32)
     ulps = np.abs(x.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
     assert ulps.max() <= 1 and (ulps == 0).mean() > 0.999
+
+
+def test_fft2_oracle_pinned():
+    # against numpy's complex128 fft2 (pocketfft, an independent library), the
+    # closed form of a shifted impulse, and the O(N^2) direct DFT on 8 x 16
+    rng = np.random.default_rng(5)
+    for n1, n2, b in ((16, 32, 2), (64, 16, 1), (8, 8, 3)):
+        x = rng.uniform(-1, 1, (b, n1, 2 * n2)).astype(np.float32)
+        got = oracle.fft2_f64(x, n1, n2).reshape(b, n1, n2, 2)
+        z = x.reshape(b, n1, n2, 2).astype(np.float64)
+        want = np.fft.fft2(z[..., 0] + 1j * z[..., 1])
+        assert np.abs(got[..., 0] + 1j * got[..., 1] - want).max() <= 1e-12 * np.abs(want).max()
+        inv = oracle.fft2_f64(x, n1, n2, sign=1).reshape(b, n1, n2, 2)
+        want = np.fft.ifft2(z[..., 0] + 1j * z[..., 1]) * (n1 * n2)
+        assert np.abs(inv[..., 0] + 1j * inv[..., 1] - want).max() <= 1e-12 * np.abs(want).max()
+    n1, n2, a, c = 32, 64, 5, 9
+    x = np.zeros((n1, 2 * n2), np.float32)
+    x[a, 2 * c] = 1.0
+    got = oracle.fft2_f64(x, n1, n2).reshape(n1, n2, 2)
+    k1, k2 = np.meshgrid(np.arange(n1), np.arange(n2), indexing="ij")
+    want = np.exp(-2j * np.pi * (a * k1 / n1 + c * k2 / n2))
+    assert np.abs(got[..., 0] + 1j * got[..., 1] - want).max() < 1e-12
+    n1, n2 = 8, 16
+    x = rng.uniform(-1, 1, (n1, 2 * n2)).astype(np.float32)
+    F1 = np.exp(-2j * np.pi * np.outer(np.arange(n1), np.arange(n1)) / n1)
+    F2 = np.exp(-2j * np.pi * np.outer(np.arange(n2), np.arange(n2)) / n2)
+    z = x.reshape(n1, n2, 2).astype(np.float64)
+    want = F1 @ (z[..., 0] + 1j * z[..., 1]) @ F2
+    got = oracle.fft2_f64(x, n1, n2).reshape(n1, n2, 2)
+    assert np.abs(got[..., 0] + 1j * got[..., 1] - want).max() < 1e-11
