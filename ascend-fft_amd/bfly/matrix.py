@@ -100,7 +100,7 @@ def shell_line(case, reps, exe=DEFAULT_EXE):
     return " ".join(f"{k}={v}" for k, v in sorted(env.items())) + " " + " ".join(argv)
 
 
-def judge(returncode, output):
+def verdict_of(returncode, output):
     """(max_rel, correct) of one bfly_check run: exit 0, a PASS line and a
     finite maxRel in [0, 1e-4]."""
     m = re.search(r"maxRel=([0-9.eE+-]+|nan|inf)", output)
@@ -123,7 +123,7 @@ def run_case(case, reps, trial, exe=DEFAULT_EXE, timeout=600.0):
                     failure=f"timeout after {timeout:.0f} s", output_tail=str(exc))
     wall_ms = (time.perf_counter() - t0) * 1e3
     out = r.stdout + r.stderr
-    err, ok = judge(r.returncode, out)
+    err, ok = verdict_of(r.returncode, out)
     m = re.search(r"device_chain=([0-9.]+) us", out)
     return dict(case, trial=trial, returncode=r.returncode, max_rel=err, wall_ms=wall_ms,
                 device_us=float(m.group(1)) if m else math.nan, correct=ok,

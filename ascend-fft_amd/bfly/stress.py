@@ -38,7 +38,7 @@ PKG = Path(__file__).resolve().parents[1]
 if str(PKG) not in sys.path:
     sys.path.insert(0, str(PKG))
 
-from bfly.matrix import DEFAULT_CONFIG, DEFAULT_EXE, command_for, judge, load_config  # noqa: E402
+from bfly.matrix import DEFAULT_CONFIG, DEFAULT_EXE, command_for, load_config, verdict_of  # noqa: E402
 
 MIB = 1 << 20
 GIB = 1 << 30
@@ -240,7 +240,7 @@ def execute_case(row, exe=DEFAULT_EXE):
     extra, argv = command_for(case, row["reps"], exe)
     timeout_s = case_timeout_s(row["reps"])
     rc, out, samples, timed_out, wall_ms = run_with_rss(argv, _env(extra), timeout_s)
-    err, ok = judge(rc, out)
+    err, ok = verdict_of(rc, out)
     failure = f"timeout after {timeout_s:.0f} seconds" if timed_out else (
         "" if ok else "exit status, PASS line or maxRel check failed")
     ok = ok and not timed_out
@@ -271,7 +271,7 @@ def run_lifecycle(row, cycles, exe=DEFAULT_EXE):
         except subprocess.TimeoutExpired:
             failed.append(cycle)
             continue
-        if not judge(r.returncode, r.stdout + r.stderr)[1]:
+        if not verdict_of(r.returncode, r.stdout + r.stderr)[1]:
             failed.append(cycle)
     row.update(reps=cycles, returncode=0 if not failed else 1,
                wall_ms=round((time.perf_counter() - t0) * 1e3, 1), correct=not failed,

@@ -109,7 +109,7 @@ def cpu_baseline(budget_s: float):
 
 
 def secondary_rows(cpu_seconds: float, reps: int = 50):
-    """The other hot-path rows at their BASELINE shapes, inputs resident and
+    """The other hot-path rows at the shapes the reference quotes, inputs resident and
     rotated over 4 buffer sets (>= 256 MB MALL): cfg1 C2C latency, R2C / C2R
     4096x4096 (with the reference's refR2CF32 / refC2RF32 timed on one host
     core over a bounded sample), long C2C at 65536 (b = 64 and 256) and
@@ -161,10 +161,11 @@ def secondary_rows(cpu_seconds: float, reps: int = 50):
     hs = [torch.empty(b, n + 2, device=dev) for _ in range(4)]
     for kind, name, src, dst in ((1, "r2c_n4096_b4096", xr, hs), (2, "c2r_n4096_b4096", hs, xr)):
         tab = ops.table(kind, n, dev)
-        fn = (lambda k, t, a, c: lambda i: _lib.check(lib.bfly_real_ex(
-            k, a[i % 4].data_ptr(), c[i % 4].data_ptr(), n, b, n // 2 + 1, n + 2, _lib.VARIANT_AUTO, 0,
-            t.data_ptr(), sid()), "real"))(kind, tab, src, dst)
-        out[name] = row(timed(fn), (4 * n + 4 * (n + 2)) * b, b)
+        # arguments prepared once: the host loop must outpace the ~27 us kernels
+        args = [(kind, src[i].data_ptr(), dst[i].data_ptr(), n, b, n // 2 + 1, n + 2, _lib.VARIANT_AUTO, 0,
+                 tab.data_ptr(), sid()) for i in range(4)]
+        out[name] = row(timed(lambda i: _lib.check(lib.bfly_real_ex(*args[i % 4]), "real")),
+                        (4 * n + 4 * (n + 2)) * b, b)
     del xr, hs
     r = oracle.ref()
     if r is not None and cpu_seconds > 0:

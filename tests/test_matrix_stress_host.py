@@ -53,11 +53,11 @@ def test_commands_and_verdicts(stub_cli):
     assert argv[1:] == ["1024", "3", "5"]
     half = dict(case, direction="c2r", input="half-spectrum", seed=0)
     assert matrix.command_for(half, 1)[0] == {"AB_DIR": "c2r"}
-    assert matrix.judge(0, "n=1 maxRel=2.000e-07 (worst idx 0)\nPASS\n") == (2e-7, True)
-    assert matrix.judge(0, "maxRel=2.000e-04\nPASS\n")[1] is False       # above the gate
-    assert matrix.judge(1, "maxRel=2.000e-07\nFAIL\n")[1] is False
-    assert matrix.judge(0, "maxRel=nan\nPASS\n")[1] is False
-    assert matrix.judge(0, "PASS\n")[1] is False                        # no error figure
+    assert matrix.verdict_of(0, "n=1 maxRel=2.000e-07 (worst idx 0)\nPASS\n") == (2e-7, True)
+    assert matrix.verdict_of(0, "maxRel=2.000e-04\nPASS\n")[1] is False       # above the gate
+    assert matrix.verdict_of(1, "maxRel=2.000e-07\nFAIL\n")[1] is False
+    assert matrix.verdict_of(0, "maxRel=nan\nPASS\n")[1] is False
+    assert matrix.verdict_of(0, "PASS\n")[1] is False                        # no error figure
 
 
 def test_profile_run_reports_failures(stub_cli, tmp_path, capsys):
