@@ -143,16 +143,17 @@ hipError_t fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t
     const float2* tc = reinterpret_cast<const float2*>(table_cols);
     const bfly_host::Fft2dLayout L = bfly_host::fft2d_layout(n1);
     const uint64_t item = (uint64_t)n1 * n2;
+    // the row pass takes the C2C kernel the library picks for (n2, batch * n1)
+    const int rows = bfly_host::auto_variant(n2, batch * n1);
     if (L.q == 1) {
         // rows in -> ws, columns ws -> out
-        hipError_t e = c2c(in, reinterpret_cast<float*>(ws), n2, batch * n1, sign, BFLY_VARIANT_AUTO, table_rows,
-                           nullptr, s);
+        hipError_t e = c2c(in, reinterpret_cast<float*>(ws), n2, batch * n1, sign, rows, table_rows, nullptr, s);
         if (e != hipSuccess) return e;
         ColView cv{item, 0, 0, n2, n2, n2, 0, 1, 0};
         return cols<false>(n1, ws, o2, tc + L.p_table, nullptr, cv, batch, sign, s);
     }
     // rows in -> out, columns (P-point, twiddle) out -> ws, (Q-point, row map) ws -> out
-    hipError_t e = c2c(in, out, n2, batch * n1, sign, BFLY_VARIANT_AUTO, table_rows, nullptr, s);
+    hipError_t e = c2c(in, out, n2, batch * n1, sign, rows, table_rows, nullptr, s);
     if (e != hipSuccess) return e;
     uint32_t shift = 0;
     while ((1u << shift) < n2) shift++;
