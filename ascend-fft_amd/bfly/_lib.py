@@ -61,7 +61,8 @@ SYMBOLS = (
     "bfly_rt_malloc", "bfly_rt_mem_info", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
     "bfly_rt_memcpy_h2d_async", "bfly_rt_memcpy_d2h_async", "bfly_rt_malloc_host", "bfly_rt_free_host",
     "bfly_rt_event_create", "bfly_rt_event_destroy", "bfly_rt_event_record", "bfly_rt_event_elapsed",
-    "bfly_c2c_launches", "bfly_c2c_traced", "bfly_fft2d_table_floats", "bfly_fft2d_table_fill",
+    "bfly_c2c_launches", "bfly_c2c_traced", "bfly_c2c_columns_workspace_bytes", "bfly_c2c_columns",
+    "bfly_fft2d_table_floats", "bfly_fft2d_table_fill",
     "bfly_fft2d_workspace_bytes", "bfly_fft2d", "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
     "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
@@ -109,6 +110,8 @@ def _bind(lib):
         "bfly_rt_event_elapsed": (i32, [vp, vp, C.POINTER(C.c_float)]),
         "bfly_c2c_launches": (i32, [u32, u32, i32]),
         "bfly_c2c_traced": (i32, [vp, vp, u32, u32, i32, i32, vp, vp, vp, C.POINTER(vp), i32]),
+        "bfly_c2c_columns_workspace_bytes": (sz, [u32, u32, u32]),
+        "bfly_c2c_columns": (i32, [vp, vp, u32, u32, u32, i32, vp, vp, vp]),
         "bfly_fft2d_table_floats": (sz, [u32]),
         "bfly_fft2d_table_fill": (i32, [u32, vp]),
         "bfly_fft2d_workspace_bytes": (sz, [u32, u32, u32]),

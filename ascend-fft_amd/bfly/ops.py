@@ -118,6 +118,27 @@ def fft2_raw(x: torch.Tensor, out: torch.Tensor, n1: int, n2: int, batch: int, s
     _lib.check(rc, f"bfly_fft2d(n1={n1}, n2={n2}, batch={batch}, sign={sign})")
 
 
+def fft_columns(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
+    """Unnormalised C2C along axis -2 of a complex64 tensor [..., n, ncols]
+    (element stride ncols) = torch.fft.fft(x, dim=-2) for sign -1."""
+    _check_device(x, "x")
+    if x.dtype != torch.complex64 or x.dim() < 2:
+        raise TypeError("fft_columns takes a complex64 tensor [..., n, ncols]")
+    n, ncols = x.shape[-2], x.shape[-1]
+    batch = x.numel() // (n * ncols) if n * ncols else 0
+    out = torch.empty_like(x)
+    if batch == 0:
+        return out
+    lib = _lib.hip()
+    nbytes = lib.bfly_c2c_columns_workspace_bytes(n, ncols, batch)
+    work = torch.empty(nbytes // 4, dtype=torch.float32, device=x.device) if nbytes else None
+    rc = lib.bfly_c2c_columns(x.data_ptr(), out.data_ptr(), n, ncols, batch, sign,
+                              table2d_cols(n, x.device).data_ptr(), work.data_ptr() if work is not None else None,
+                              _stream(x.device))
+    _lib.check(rc, f"bfly_c2c_columns(n={n}, ncols={ncols}, batch={batch}, sign={sign})")
+    return out
+
+
 def fft2(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
     """Unnormalised 2-D C2C over the last two axes of a complex64 tensor
     [..., n1, n2] (sign -1 forward, +1 inverse) = torch.fft.fft2 (norm

@@ -136,6 +136,16 @@ int bfly_rt_event_elapsed(void* start, void* end, float* ms) {
     return hipEventElapsedTime(ms, a, b) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
 }
 
+size_t bfly_c2c_columns_workspace_bytes(uint32_t n, uint32_t ncols, uint32_t batch) {
+    return bfly_host::fft2d_layout(n).q > 1 ? 8 * (size_t)n * ncols * batch : 0;
+}
+
+int bfly_c2c_columns(const float* in, float* out, uint32_t n, uint32_t ncols, uint32_t batch, int sign,
+                     const float* table_cols, void* work, void* stream) {
+    if (int g = bfly_host::check_columns(sign, in, out, n, ncols, batch, table_cols, work)) return g;
+    return launched(bfly_kern::columns(in, out, n, ncols, batch, sign, table_cols, work, S(stream)));
+}
+
 size_t bfly_fft2d_table_floats(uint32_t n1) { return 2 * bfly_host::fft2d_layout(n1).complex_len; }
 
 int bfly_fft2d_table_fill(uint32_t n1, float* host_table) { return bfly_host::fft2d_fill(n1, host_table); }

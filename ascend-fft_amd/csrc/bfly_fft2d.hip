@@ -40,7 +40,7 @@ struct ColView {
     uint64_t in_mstride, out_mstride;
     uint32_t in_rstride, out_rstride;
     uint32_t ncols, groups, mats;
-    uint32_t col_shift;  // twiddle pass: q = column >> col_shift
+    uint32_t col_div;  // twiddle pass: q = column / col_div
 };
 
 template <int I, int DIR, bool TW, int WG = bfly_plan::kBlock>
@@ -83,7 +83,7 @@ __global__ __launch_bounds__(WG) void k_cols(const float2* __restrict__ in, floa
                                    std::make_index_sequence<PL::NRAD>{});
     if (!active) return;
     float2* dst = out + b * cv.item + m * cv.out_mstride + col;
-    const uint32_t q = TW ? col >> cv.col_shift : 0u;
+    const uint32_t q = TW ? col / cv.col_div : 0u;
 #pragma unroll
     for (int mm = 0; mm < P / RL; mm++)
 #pragma unroll
@@ -135,34 +135,40 @@ hipError_t cols(uint32_t len, const float2* in, float2* out, const float2* tw, c
 
 namespace bfly_kern {
 
-hipError_t fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign,
-                 const float* table_rows, const float* table_cols, void* work, hipStream_t s) {
+hipError_t columns(const float* in, float* out, uint32_t n1, uint32_t ncols, uint32_t batch, int sign,
+                   const float* table_cols, void* work, hipStream_t s) {
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     float2* ws = reinterpret_cast<float2*>(work);
     const float2* tc = reinterpret_cast<const float2*>(table_cols);
     const bfly_host::Fft2dLayout L = bfly_host::fft2d_layout(n1);
-    const uint64_t item = (uint64_t)n1 * n2;
+    const uint64_t item = (uint64_t)n1 * ncols;
+    if (L.q == 1) {
+        ColView cv{item, 0, 0, ncols, ncols, ncols, 0, 1, 1};
+        return cols<false>(n1, i2, o2, tc + L.p_table, nullptr, cv, batch, sign, s);
+    }
+    // P-point pass with the twiddle: in -> ws; Q-point pass with the row map: ws -> out
+    const uint32_t qn = L.q * ncols;
+    ColView a{item, 0, 0, qn, qn, qn, 0, 1, ncols};
+    hipError_t e = cols<true>(L.p, i2, ws, tc + L.p_table, tc + L.wab, a, batch, sign, s);
+    if (e != hipSuccess) return e;
+    ColView b{item, qn, ncols, ncols, L.p * ncols, ncols, 0, L.p, 1};
+    return cols<false>(L.q, ws, o2, tc + L.q_table, nullptr, b, batch, sign, s);
+}
+
+hipError_t fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign,
+                 const float* table_rows, const float* table_cols, void* work, hipStream_t s) {
     // the row pass takes the C2C kernel the library picks for (n2, batch * n1)
     const int rows = bfly_host::auto_variant(n2, batch * n1);
-    if (L.q == 1) {
-        // rows in -> ws, columns ws -> out
-        hipError_t e = c2c(in, reinterpret_cast<float*>(ws), n2, batch * n1, sign, rows, table_rows, nullptr, s);
-        if (e != hipSuccess) return e;
-        ColView cv{item, 0, 0, n2, n2, n2, 0, 1, 0};
-        return cols<false>(n1, ws, o2, tc + L.p_table, nullptr, cv, batch, sign, s);
+    if (bfly_host::fft2d_layout(n1).q == 1) {
+        // rows in -> work, columns work -> out
+        float* w = static_cast<float*>(work);
+        hipError_t e = c2c(in, w, n2, batch * n1, sign, rows, table_rows, nullptr, s);
+        return e != hipSuccess ? e : columns(w, out, n1, n2, batch, sign, table_cols, nullptr, s);
     }
-    // rows in -> out, columns (P-point, twiddle) out -> ws, (Q-point, row map) ws -> out
+    // rows in -> out, columns out -> work -> out
     hipError_t e = c2c(in, out, n2, batch * n1, sign, rows, table_rows, nullptr, s);
-    if (e != hipSuccess) return e;
-    uint32_t shift = 0;
-    while ((1u << shift) < n2) shift++;
-    const uint32_t qn2 = L.q * n2;
-    ColView a{item, 0, 0, qn2, qn2, qn2, 0, 1, shift};
-    e = cols<true>(L.p, o2, ws, tc + L.p_table, tc + L.wab, a, batch, sign, s);
-    if (e != hipSuccess) return e;
-    ColView b{item, qn2, n2, n2, L.p * n2, n2, 0, L.p, 0};
-    return cols<false>(L.q, ws, o2, tc + L.q_table, nullptr, b, batch, sign, s);
+    return e != hipSuccess ? e : columns(out, out, n1, n2, batch, sign, table_cols, work, s);
 }
 
 }  // namespace bfly_kern

@@ -28,6 +28,10 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
 hipError_t real(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,
                 uint32_t stride, int variant, bool adjoint, const float* table, hipStream_t s);
 hipError_t copy(const void* src, void* dst, size_t bytes, hipStream_t s);
+// C2C along the first axis of [batch][n1][ncols] (bfly_fft2d.hip); `work`
+// holds n1*ncols*batch complex values when n1 > 256 (unused otherwise).
+hipError_t columns(const float* in, float* out, uint32_t n1, uint32_t ncols, uint32_t batch, int sign,
+                   const float* table_cols, void* work, hipStream_t s);
 // 2-D C2C of [batch][n1][n2] (bfly_fft2d.hip); `work` holds n1*n2*batch
 // complex values.
 hipError_t fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t batch, int sign,

@@ -122,6 +122,24 @@ int fft2d_fill(uint32_t n1, float* dst) {
     return BFLY_OK;
 }
 
+int check_columns(int sign, const void* in, const void* out, uint32_t n1, uint32_t ncols, uint32_t batch,
+                  const void* table_cols, const void* work) {
+    const Fft2dLayout L = fft2d_layout(n1);
+    if (!L.complex_len || ncols == 0 || batch == 0 || !in || !out) return BFLY_ERR_SHAPE;
+    if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
+    if (!table_cols) return BFLY_ERR_TABLE;
+    if (L.q > 1 && !work) return BFLY_ERR_SHAPE;
+    const uint64_t item = (uint64_t)n1 * ncols;
+    // widest column grid: batch * q * ceil(p * ncols / 16) workgroups at most
+    if ((uint64_t)batch * item / 16 + (uint64_t)batch * L.p > 0x7fffffffull) return BFLY_ERR_SHAPE;
+    const uint64_t bytes = 8 * item * batch;
+    const auto lo = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+    const auto apart = [&](const void* a, const void* b) { return lo(a) + bytes <= lo(b) || lo(b) + bytes <= lo(a); };
+    if (!apart(in, out)) return BFLY_ERR_SHAPE;
+    if (L.q > 1 && (!apart(in, work) || !apart(out, work))) return BFLY_ERR_SHAPE;
+    return BFLY_OK;
+}
+
 int check_fft2d(int sign, const void* in, const void* out, uint32_t n1, uint32_t n2, uint32_t batch,
                 const void* table_rows, const void* table_cols, const void* work) {
     if (!fft2d_ok(n1, n2) || batch == 0 || !in || !out || !work) return BFLY_ERR_SHAPE;

@@ -61,6 +61,10 @@ struct Fft2dLayout {
 bool fft2d_ok(uint32_t n1, uint32_t n2);
 Fft2dLayout fft2d_layout(uint32_t n1);
 int fft2d_fill(uint32_t n1, float* dst);
+// Guards of bfly_c2c_columns (n1 a power of two in [16, 4096], ncols and
+// batch >= 1, disjoint buffers, a workspace when n1 > 256, grids that fit).
+int check_columns(int sign, const void* in, const void* out, uint32_t n1, uint32_t ncols, uint32_t batch,
+                  const void* table_cols, const void* work);
 // Guards of bfly_fft2d: BFLY_OK or the status code.
 int check_fft2d(int sign, const void* in, const void* out, uint32_t n1, uint32_t n2, uint32_t batch,
                 const void* table_rows, const void* table_cols, const void* work);

@@ -262,6 +262,24 @@ def test_fft2d_guards_through_stub():
     assert s.bfly_fft2d(i, i + 8 * 64 * 64, 64, 64, 1, -1, t, tc, w, None) == 0  # adjacent
 
 
+def test_strided_columns_guards_through_stub():
+    s = bfly.stub()
+    buf = np.zeros(16, np.float32)
+    p = buf.ctypes.data
+    i, o, w, t = p, p + (1 << 30), p + (2 << 30), p + (3 << 30)
+    assert s.bfly_c2c_columns(i, o, 256, 100, 3, -1, t, None, None) == 0      # one pass: no workspace
+    assert s.bfly_c2c_columns_workspace_bytes(256, 100, 3) == 0
+    assert s.bfly_c2c_columns_workspace_bytes(512, 100, 3) == 8 * 512 * 100 * 3
+    assert s.bfly_c2c_columns(i, o, 512, 100, 3, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_columns(i, o, 512, 100, 3, 1, t, w, None) == 0
+    for n, ncols, b in ((8, 4, 1), (8192, 4, 1), (96, 4, 1), (64, 0, 1), (64, 4, 0)):
+        assert s.bfly_c2c_columns(i, o, n, ncols, b, -1, t, w, None) == bfly.ERR_SHAPE, (n, ncols, b)
+    assert s.bfly_c2c_columns(i, o, 64, 4, 1, 2, t, w, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_columns(i, o, 64, 4, 1, -1, None, w, None) == _lib.ERR_TABLE
+    assert s.bfly_c2c_columns(i, i + 8, 64, 4, 1, -1, t, w, None) == bfly.ERR_SHAPE       # overlap
+    assert s.bfly_c2c_columns(i, o, 1024, 4, 1, -1, t, o + 64, None) == bfly.ERR_SHAPE    # work inside out
+
+
 def test_python_modules_import_and_variant_ids_match_header():
     import importlib
     for m in ("bfly.ops", "bfly.tune", "bfly.autograd", "bfly.spectral"):

@@ -86,3 +86,28 @@ def test_fft2_round_trip_and_guards(dev):
     assert lib.bfly_fft2d(p, p + 8, 64, 64, 1, -1, t, tc, p + (2 << 19), s) == _lib.ERR_SHAPE
     assert lib.bfly_fft2d(p, p + (1 << 19), 64, 64, 1, 0, t, tc, p + (2 << 19), s) == _lib.ERR_SHAPE
     assert lib.bfly_fft2d(p, p + (1 << 19), 64, 64, 1, -1, None, tc, p + (2 << 19), s) == _lib.ERR_TABLE
+
+
+COLS = [(16, 1, 3), (64, 3, 5), (128, 1000, 1), (256, 100, 2), (512, 7, 4), (1024, 1000, 1), (2048, 64, 1),
+        (4096, 33, 2), (4096, 1, 1)]
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+@pytest.mark.parametrize("n,ncols,batch", COLS, ids=[f"{a}x{b}-b{c}" for a, b, c in COLS])
+def test_strided_columns_match_the_oracle(dev, n, ncols, batch, sign):
+    import torch
+    from bfly import ops
+    print(f"columns n={n} ncols={ncols} batch={batch} sign={sign}", flush=True)
+    rng = np.random.default_rng(n + 31 * ncols + batch)
+    z = rng.uniform(-1, 1, (batch, n, ncols, 2)).astype(np.float32)
+    x = torch.from_numpy(z[..., 0] + 1j * z[..., 1]).to(torch.complex64).to(dev)
+    y = ops.fft_columns(x, sign)
+    torch.cuda.synchronize()
+    # each column as an interleaved row for the oracle
+    rows = np.ascontiguousarray(z.transpose(0, 2, 1, 3)).reshape(batch * ncols, 2 * n)
+    want = oracle.c2c_f64(rows, n, sign)
+    got = np.ascontiguousarray(y.cpu().numpy().transpose(0, 2, 1)).view(np.float32).reshape(batch * ncols, 2 * n)
+    assert oracle.rel_l2_rows(got, want) <= REL_L2
+    if sign < 0:
+        ref = torch.fft.fft(x, dim=-2)
+        assert (torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref)).item() < 3e-6
