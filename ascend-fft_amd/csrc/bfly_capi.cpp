@@ -163,6 +163,9 @@ int bfly_fft2d(const float* in, float* out, uint32_t n1, uint32_t n2, uint32_t b
 int bfly_c2c_launches(uint32_t n, uint32_t batch, int variant) {
     if (!bfly_supported(BFLY_KIND_C2C, n) || batch == 0) return BFLY_ERR_SHAPE;
     if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    if ((variant == BFLY_VARIANT_SPLIT_MFMA || variant == BFLY_VARIANT_SPLIT_MFMA_STREAM) &&
+        !bfly_host::split_mfma_ok(n))
+        return BFLY_ERR_SHAPE;
     return bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant) ? 2 : 1;
 }
 

@@ -30,6 +30,8 @@
 //   AB_VARIANT=<name|id>    gfx950 kernel variant (auto, plain, stream,
 //                           plain_swz, stream_swz, stream_lt, stream2,
 //                           four_step, split_mfma, split_mfma_stream)
+//   AB_DESC=1               long chains: print the lowering descriptor of the
+//                           chain this call runs (include/butterfly/descriptors.hpp)
 //
 // Lengths: C2C n a power of two in [16, 65536]; R2C / C2R in [32, 8192] --
 // a superset of the reference's envelope (C2C 64..65536, R2C 128..8192,
@@ -50,6 +52,7 @@
 #include <vector>
 
 #include "bfly_hip.h"
+#include "butterfly/descriptors.hpp"
 #include "butterfly/reference.hpp"
 #include "inputs.hpp"
 
@@ -250,6 +253,44 @@ int main(int argc, char** argv) {
     if (launches < 0) {
         std::printf("AB_VARIANT %d does not apply to c2c n=%u batch=%u\n", variant, n, batch);
         return 2;
+    }
+
+    // long chains: the lowering of the chain this call runs, and its split
+    // (src/host/fft_check.cpp:318-362)
+    if (is_long) {
+        const auto& hw = butterfly::builtin_hardware_profile();
+        const butterfly::TransformSpec spec{n, batch, butterfly::Precision::FP32, butterfly::Direction::C2C_FWD};
+        uint32_t n1 = n, n2 = 1;
+        if (launches == 2) {
+            uint32_t lg = 0;
+            while ((1u << lg) < n) lg++;
+            n1 = 1u << (lg / 2);
+            n2 = n / n1;
+        }
+        const auto low = launches == 2
+                             ? butterfly::query_lowering(spec, butterfly::default_long_mapping(n1, n2, hw),
+                                                         butterfly::local_fft_capability(), hw)
+                             : butterfly::query_lowering(spec, butterfly::single_pass_mapping(n, hw),
+                                                         butterfly::single_fft_capability(), hw);
+        if (!low.supported) {
+            std::printf("lowering rejected: %s\n", low.reason.c_str());
+            return 2;
+        }
+        if (env("AB_DESC")) {
+            std::printf("lowering: launches=%d gm_boundaries=%d modeled_payload_gm=%llu impl=%s resident=%s "
+                        "on_chip=%d host_assisted=%d abstract=%d kinds=",
+                        low.visible_launches, low.materialized_gm_boundaries,
+                        (unsigned long long)low.modeled_payload_gm_rw_bytes, launches == 2 ? "fused2" : "single",
+                        butterfly::residence_name(low.resident_subgraph), (int)low.whole_transform_on_chip,
+                        (int)low.host_assisted, (int)low.abstract_feasible);
+            for (size_t i = 0; i < low.launch_manifest.size(); i++)
+                std::printf("%s%s", i ? "|" : "", butterfly::launch_kind_name(low.launch_manifest[i].kind));
+            std::printf("\n");
+        }
+        const uint64_t gm = (16ull + (launches == 2 ? 8ull : 0ull)) * n * batch;
+        std::printf("long backend: n=%u split N1=%u N2=%u rows1=%llu rows2=%llu gm_bytes=%llu\n", n, n1, n2,
+                    (unsigned long long)n2 * batch, launches == 2 ? (unsigned long long)n1 * batch : 0ull,
+                    (unsigned long long)gm);
     }
 
     const size_t in_elems = kind == BFLY_KIND_C2C ? 2 * (size_t)n * batch

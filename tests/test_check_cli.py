@@ -132,6 +132,16 @@ def test_report_lines_follow_the_vocabulary(cli):
         assert scopes.infer_mode(rep["scopes"]) == "long_device"
     rep = scopes.parse_report(run(cli, 8192, 3, 2, AB_VARIANT="four_step").stdout)
     assert rep["boundary_impl"] == "fused2" and rep["launches"] == 2
+    # the lowering descriptor of the chain (include/butterfly/descriptors.hpp) and its split
+    out = run(cli, 65536, 2, 1, AB_DESC=1).stdout
+    assert "lowering: launches=2 gm_boundaries=1 modeled_payload_gm=4194304 impl=fused2" in out
+    assert "kinds=row_fft|row_fft" in out
+    assert "long backend: n=65536 split N1=256 N2=256 rows1=512 rows2=512 gm_bytes=3145728" in out
+    out = run(cli, 16384, 300, 1, AB_DESC=1).stdout
+    assert "lowering: launches=1 gm_boundaries=0" in out and "resident=block-resident on_chip=1" in out
+    assert "long backend: n=16384 split N1=16384 N2=1 rows1=300 rows2=0" in out
+    assert "lowering:" not in run(cli, 65536, 2, 1).stdout          # only with AB_DESC
+    assert run(cli, 64, 2, 1, AB_VARIANT="split_mfma").returncode == 2   # variant does not apply
     r = run(cli, 4096, 2, 2, AB_E2E=3, AB_E2E_MODE="xfer")
     assert "E2E n=4096 batch=2 reps=3" in r.stdout and "mode=xfer host=pinned" in r.stdout
     assert "E2E transfers: in=1 out=1 boundary=0 per_execution" in r.stdout

@@ -16,7 +16,7 @@ PROFILES = CONFIG["profiles"]
 
 
 def test_profiles_and_schema(tmp_path):
-    assert {"current", "smoke", "regression", "long", "stress", "fft2d"} <= set(PROFILES)
+    assert {"current", "smoke", "regression", "publication", "long", "stress", "fft2d"} <= set(PROFILES)
     p = tmp_path / "m.json"
     p.write_text(json.dumps({"schema_version": 2, "profiles": {}}))
     with pytest.raises(ValueError):
