@@ -61,6 +61,7 @@ SYMBOLS = (
     "bfly_rt_malloc", "bfly_rt_mem_info", "bfly_rt_free", "bfly_rt_memcpy_h2d", "bfly_rt_memcpy_d2h",
     "bfly_rt_memcpy_h2d_async", "bfly_rt_memcpy_d2h_async", "bfly_rt_malloc_host", "bfly_rt_free_host",
     "bfly_rt_event_create", "bfly_rt_event_destroy", "bfly_rt_event_record", "bfly_rt_event_elapsed",
+    "bfly_graph_begin", "bfly_graph_end", "bfly_graph_launch", "bfly_graph_destroy",
     "bfly_c2c_launches", "bfly_c2c_traced", "bfly_c2c_columns_workspace_bytes", "bfly_c2c_columns",
     "bfly_fft2d_table_floats", "bfly_fft2d_table_fill",
     "bfly_fft2d_workspace_bytes", "bfly_fft2d", "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
@@ -108,6 +109,10 @@ def _bind(lib):
         "bfly_rt_event_destroy": (i32, [vp]),
         "bfly_rt_event_record": (i32, [vp, vp]),
         "bfly_rt_event_elapsed": (i32, [vp, vp, C.POINTER(C.c_float)]),
+        "bfly_graph_begin": (i32, [vp]),
+        "bfly_graph_end": (i32, [vp, C.POINTER(vp)]),
+        "bfly_graph_launch": (i32, [vp, vp]),
+        "bfly_graph_destroy": (i32, [vp]),
         "bfly_c2c_launches": (i32, [u32, u32, i32]),
         "bfly_c2c_traced": (i32, [vp, vp, u32, u32, i32, i32, vp, vp, vp, C.POINTER(vp), i32]),
         "bfly_c2c_columns_workspace_bytes": (sz, [u32, u32, u32]),

@@ -110,6 +110,43 @@ int bfly_rt_free_host(void* ptr) {
     return hipHostFree(ptr) == hipSuccess ? BFLY_OK : BFLY_ERR_ALLOC;
 }
 
+int bfly_graph_begin(void* stream) {
+    if (!stream) return BFLY_ERR_SHAPE;
+    return hipStreamBeginCapture(S(stream), hipStreamCaptureModeThreadLocal) == hipSuccess ? BFLY_OK
+                                                                                          : BFLY_ERR_LAUNCH;
+}
+
+int bfly_graph_end(void* stream, void** graph) {
+    if (!stream || !graph) return BFLY_ERR_SHAPE;
+    *graph = nullptr;
+    hipGraph_t g = nullptr;
+    if (hipStreamEndCapture(S(stream), &g) != hipSuccess || !g) {
+        (void)hipGetLastError();
+        if (g) (void)hipGraphDestroy(g);
+        return BFLY_ERR_LAUNCH;
+    }
+    hipGraphExec_t x = nullptr;
+    const hipError_t e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (e != hipSuccess) {
+        (void)hipGetLastError();
+        return BFLY_ERR_ALLOC;
+    }
+    *graph = x;
+    return BFLY_OK;
+}
+
+int bfly_graph_launch(void* graph, void* stream) {
+    if (!graph) return BFLY_ERR_SHAPE;
+    return hipGraphLaunch(reinterpret_cast<hipGraphExec_t>(graph), S(stream)) == hipSuccess ? BFLY_OK
+                                                                                             : BFLY_ERR_LAUNCH;
+}
+
+int bfly_graph_destroy(void* graph) {
+    if (!graph) return BFLY_OK;
+    return hipGraphExecDestroy(reinterpret_cast<hipGraphExec_t>(graph)) == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC;
+}
+
 int bfly_rt_event_create(void** event) {
     if (!event) return BFLY_ERR_SHAPE;
     hipEvent_t e = nullptr;

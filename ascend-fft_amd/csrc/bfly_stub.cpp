@@ -103,6 +103,18 @@ int bfly_rt_memcpy_d2h_async(void* dst, const void* src, size_t bytes, void* str
 }
 int bfly_rt_malloc_host(void** ptr, size_t bytes) { return bfly_rt_malloc(ptr, bytes); }
 int bfly_rt_free_host(void* ptr) { return bfly_rt_free(ptr); }
+// graphs: begin/end bracket nothing; a launch counts as one
+int bfly_graph_begin(void* stream) { return stream ? BFLY_OK : BFLY_ERR_SHAPE; }
+int bfly_graph_end(void* stream, void** graph) {
+    if (!stream || !graph) return BFLY_ERR_SHAPE;
+    *graph = new int(0);
+    return BFLY_OK;
+}
+int bfly_graph_launch(void* graph, void*) { return graph ? launch() : BFLY_ERR_SHAPE; }
+int bfly_graph_destroy(void* graph) {
+    delete static_cast<int*>(graph);
+    return BFLY_OK;
+}
 // events hold the host clock at their last record (ms)
 int bfly_rt_event_create(void** event) {
     if (!event) return BFLY_ERR_SHAPE;

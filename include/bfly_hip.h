@@ -90,6 +90,24 @@ BFLY_API int bfly_rt_time_transform(int kind, int sign, const float* in, float* 
                            uint32_t batch, const float* table, void* work, int reps,
                            void* stream, float* ms);
 
+/* ------------------------------------------------------------- hipGraph
+ * Capture of the device-pointer calls issued on `stream` into a replayable
+ * graph: bfly_graph_begin starts capturing the stream (thread-local mode;
+ * nothing executes while capturing), bfly_graph_end ends the capture and
+ * instantiates it into *graph, bfly_graph_launch replays it on a stream and
+ * bfly_graph_destroy frees it.  Capturable: bfly_c2c / bfly_c2c_ex,
+ * bfly_r2c / bfly_c2r / their _ex forms, bfly_real_ex, bfly_fft2d,
+ * bfly_c2c_columns, bfly_mode_mix, bfly_group_launch and the _async copies;
+ * not the synchronising helpers (bfly_rt_memcpy_h2d / _d2h,
+ * bfly_rt_stream_sync, bfly_rt_time_transform, bfly_rt_event_elapsed).
+ * `stream` must be a created stream, not the null stream.  No counterpart in
+ * the reference, whose Plan::run is synchronous (src/framework/butterfly.cpp:723-749);
+ * the thin C ABI's graph helpers of its Context/Plan replacement. */
+BFLY_API int bfly_graph_begin(void* stream);
+BFLY_API int bfly_graph_end(void* stream, void** graph);
+BFLY_API int bfly_graph_launch(void* graph, void* stream);
+BFLY_API int bfly_graph_destroy(void* graph);
+
 /* ---------------------------------------------------------- shape rules */
 /* 1 when the library has kernels for (kind, n), else 0.  C2C: n a power of
  * two in [16, 65536]; R2C/C2R: n a power of two in [32, 8192]. */
