@@ -28,6 +28,8 @@ int bfly_rt_init(int device) {
     return hipSetDevice(device) == hipSuccess ? BFLY_OK : BFLY_ERR_TABLE;
 }
 
+int bfly_rt_teardown(void) { return hipDeviceSynchronize() == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC; }
+
 int bfly_rt_stream_create(void** stream) {
     if (!stream) return BFLY_ERR_SHAPE;
     hipStream_t s = nullptr;
@@ -242,6 +244,16 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream) {
     return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);
+}
+
+int bfly_c2c_fwd(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
+                 void* stream) {
+    return bfly_c2c_ex(in, out, n, batch, -1, BFLY_VARIANT_AUTO, table, work, stream);
+}
+
+int bfly_c2c_inv(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
+                 void* stream) {
+    return bfly_c2c_ex(in, out, n, batch, 1, BFLY_VARIANT_AUTO, table, work, stream);
 }
 
 int bfly_real_ex(int kind, const float* in, float* out, uint32_t n, uint32_t batch, uint32_t bins,

@@ -50,6 +50,10 @@ int bfly_rt_init(int device) {
     ++g.inits;
     return device == 0 ? BFLY_OK : BFLY_ERR_TABLE;
 }
+int bfly_rt_teardown(void) {
+    ++g.syncs;
+    return BFLY_OK;
+}
 int bfly_rt_stream_create(void** stream) {
     if (!stream) return BFLY_ERR_SHAPE;
     ++g.streams_created;
@@ -195,6 +199,14 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream) {
     return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);
+}
+int bfly_c2c_fwd(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
+                 void* stream) {
+    return bfly_c2c_ex(in, out, n, batch, -1, BFLY_VARIANT_AUTO, table, work, stream);
+}
+int bfly_c2c_inv(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
+                 void* stream) {
+    return bfly_c2c_ex(in, out, n, batch, 1, BFLY_VARIANT_AUTO, table, work, stream);
 }
 int bfly_r2c(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* stream) {
     return bfly_real_ex(BFLY_KIND_R2C, in, out, n, batch, n / 2 + 1, n + 2, BFLY_VARIANT_AUTO, 0, table, stream);

@@ -56,6 +56,12 @@ extern "C" {
  * aclrtMemcpy H2D/D2H (butterfly.cpp:263-269, 355, 368),
  * aclrtSynchronizeStream (butterfly.cpp:367). */
 BFLY_API int bfly_rt_init(int device);
+/* Waits for all work the library queued on the current device; the library
+ * keeps no device allocation or stream of its own to release (the teardown
+ * half of aclFinalize / aclrtResetDevice in the reference's Context,
+ * src/framework/butterfly.cpp:552-561, without resetting a device the
+ * caller may share). */
+BFLY_API int bfly_rt_teardown(void);
 BFLY_API int bfly_rt_stream_create(void** stream);
 BFLY_API int bfly_rt_stream_destroy(void* stream);
 BFLY_API int bfly_rt_stream_sync(void* stream);
@@ -142,6 +148,11 @@ BFLY_API size_t bfly_workspace_bytes(int kind, uint32_t n, uint32_t batch);
  * butterfly.cpp:625, src/ascendc/fft_radix2.cpp:64-69). */
 BFLY_API int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream);
+/* bfly_c2c with sign -1 (forward) and +1 (unnormalised inverse). */
+BFLY_API int bfly_c2c_fwd(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
+                          void* stream);
+BFLY_API int bfly_c2c_inv(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
+                          void* stream);
 
 /* ------------------------------------------------------------------ 2-D
  * 2-D C2C of `batch` row-major [n1][n2] arrays of interleaved complex

@@ -80,3 +80,23 @@ def test_graph_replays_equal_eager_calls(dev):
     finally:
         graph.close()
     assert lib.bfly_graph_begin(None) == _lib.ERR_SHAPE
+
+
+def test_forward_inverse_wrappers_and_teardown(dev):
+    import torch
+    from bfly import _lib, ops
+    lib = _lib.hip()
+    sid = torch.cuda.current_stream(dev).cuda_stream
+    for n, b in ((1024, 7), (65536, 2)):
+        x = torch.rand(b, 2 * n, device=dev) * 2 - 1
+        t = ops.table(_lib.KIND_C2C, n, dev)
+        w = ops.workspace(_lib.KIND_C2C, n, b, dev)
+        wp = w.data_ptr() if w is not None else None
+        outs = [torch.empty_like(x) for _ in range(4)]
+        assert lib.bfly_c2c_fwd(x.data_ptr(), outs[0].data_ptr(), n, b, t.data_ptr(), wp, sid) == 0
+        assert lib.bfly_c2c(x.data_ptr(), outs[1].data_ptr(), n, b, -1, t.data_ptr(), wp, sid) == 0
+        assert lib.bfly_c2c_inv(x.data_ptr(), outs[2].data_ptr(), n, b, t.data_ptr(), wp, sid) == 0
+        assert lib.bfly_c2c(x.data_ptr(), outs[3].data_ptr(), n, b, 1, t.data_ptr(), wp, sid) == 0
+        torch.cuda.synchronize()
+        assert torch.equal(outs[0], outs[1]) and torch.equal(outs[2], outs[3])
+    assert lib.bfly_rt_teardown() == 0
