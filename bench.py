@@ -108,6 +108,44 @@ def cpu_baseline(budget_s: float):
     }
 
 
+def cpu_libraries(budget_s: float = 2.0):
+    """Host FFT libraries on the same C2C workload shape (n = 4096, fp32
+    complex rows), each over a bounded sample of rows:
+    torch.fft.fft on CPU tensors with every host thread, scipy.fft with
+    every host thread, numpy.fft on one thread.  Context for the reference's
+    own CPU figure (cpu_baseline), not a target."""
+    import os
+
+    import numpy as np
+    import scipy.fft
+    import torch
+
+    cores = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    rows = 512
+    x = np.random.default_rng(0).uniform(-1, 1, (rows, 2 * N)).astype(np.float32).view(np.complex64)
+    xt = torch.from_numpy(x)
+    old = torch.get_num_threads()
+    torch.set_num_threads(cores)
+    runs = {"torch.fft.fft (CPU)": (lambda: torch.fft.fft(xt), cores),
+            "scipy.fft.fft": (lambda: scipy.fft.fft(x, workers=cores), cores),
+            "numpy.fft.fft": (lambda: np.fft.fft(x), 1)}
+    out = {}
+    try:
+        for name, (fn, thr) in runs.items():
+            fn()
+            t0, done = time.perf_counter(), 0
+            while time.perf_counter() - t0 < budget_s:
+                fn()
+                done += rows
+            dt = time.perf_counter() - t0
+            out[name] = {"GB_per_s": round(16.0 * N * done / dt / 1e9, 3), "threads": thr,
+                         "sample": f"{done} rows of n={N} in {dt:.1f} s (the same {rows} rows, 16 MiB, "
+                                   f"reused: host-cache resident)"}
+    finally:
+        torch.set_num_threads(old)
+    return out
+
+
 def secondary_rows(cpu_seconds: float, reps: int = 50):
     """The other hot-path rows at the shapes the reference quotes, inputs resident and
     rotated over 4 buffer sets (>= 256 MB MALL): cfg1 C2C latency, R2C / C2R
@@ -621,6 +659,8 @@ def main():
         },
         "cpu_baseline": None if args.no_cpu else cpu_baseline(args.cpu_seconds),
     }
+    if not args.no_cpu:
+        line["cpu_libraries"] = cpu_libraries()
     if not args.no_patterns:
         line["rows"] = secondary_rows(0 if args.no_cpu else 1.5)
         line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern(),
