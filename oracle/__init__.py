@@ -166,6 +166,16 @@ def c2c_f64(x: np.ndarray, n: int, sign: int = -1) -> np.ndarray:
     return out
 
 
+def fft_f64(z: np.ndarray, n: int, sign: int = -1) -> np.ndarray:
+    """FP64 oracle FFT of FP64 interleaved rows [rows][2n] (a copy)."""
+    a = np.array(z, dtype=np.float64, copy=True).reshape(-1, 2 * n)
+    for r in range(a.shape[0]):
+        row = np.ascontiguousarray(a[r])
+        lib().orc_fft_f64_inplace(row, n, sign)
+        a[r] = row
+    return a
+
+
 def fft2_f64(x: np.ndarray, n1: int, n2: int, sign: int = -1) -> np.ndarray:
     """FP64 oracle 2-D C2C of interleaved fp32 arrays [batch][n1][2 n2]."""
     x = _f32(x).reshape(-1, n1 * 2 * n2)
