@@ -146,12 +146,14 @@ def cpu_libraries(budget_s: float = 2.0):
     return out
 
 
-def secondary_rows(cpu_seconds: float, reps: int = 50):
+def secondary_rows(cpu_seconds: float, reps: int = 20, rounds: int = 3):
     """The other hot-path rows at the shapes the reference quotes, inputs resident and
     rotated over 4 buffer sets (>= 256 MB MALL): cfg1 C2C latency, R2C / C2R
     4096x4096 (with the reference's refR2CF32 / refC2RF32 timed on one host
     core over a bounded sample), long C2C at 65536 (b = 64 and 256) and
-    16384 (b = 1024).  Kernel time by HIP events on the launch stream."""
+    16384 (b = 1024).  Kernel time by HIP events on the launch stream, the
+    reference's protocol (docs/benchmarks/methodology.md:52-55): warm-up,
+    then `rounds` rounds of `reps` launches, the minimum of the round means."""
     import numpy as np
     import torch
     from bfly import _lib, ops
@@ -165,13 +167,17 @@ def secondary_rows(cpu_seconds: float, reps: int = 50):
         for i in range(5):
             fn(i)
         torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for i in range(reps):
-            fn(i)
-        e1.record()
-        e1.synchronize()
-        return e0.elapsed_time(e1) / reps
+        best = None
+        for _ in range(rounds):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            for i in range(reps):
+                fn(i)
+            e1.record()
+            e1.synchronize()
+            ms = e0.elapsed_time(e1) / reps
+            best = ms if best is None else min(best, ms)
+        return best
 
     def row(ms, nbytes, transforms):
         gbs = nbytes / (ms * 1e-3) / 1e9
