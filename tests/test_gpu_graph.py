@@ -100,3 +100,53 @@ def test_forward_inverse_wrappers_and_teardown(dev):
         torch.cuda.synchronize()
         assert torch.equal(outs[0], outs[1]) and torch.equal(outs[2], outs[3])
     assert lib.bfly_rt_teardown() == 0
+
+
+def test_concurrent_host_threads_on_separate_streams(dev):
+    """Four host threads, each with its own stream, call the library at once
+    (ctypes releases the GIL during the calls): every result equals the same
+    call made alone, bit for bit (the library's host state -- resident-grid
+    caches, table layouts -- is shared and must stay consistent)."""
+    import threading
+
+    import torch
+    from bfly import _lib, ops
+    lib = _lib.hip()
+    shapes = [(4096, 256), (1024, 1024), (65536, 8), (256, 4096)]
+    g = torch.Generator(device=dev).manual_seed(11)
+    xs = [torch.rand(b, 2 * n, generator=g, device=dev) * 2 - 1 for n, b in shapes]
+    tabs = [ops.table(_lib.KIND_C2C, n, dev) for n, _ in shapes]
+    works = [ops.workspace(_lib.KIND_C2C, n, b, dev) for n, b in shapes]
+    alone = [torch.empty_like(x) for x in xs]
+    sid = torch.cuda.current_stream(dev).cuda_stream
+    for (n, b), x, y, t, w in zip(shapes, xs, alone, tabs, works):
+        assert lib.bfly_c2c(x.data_ptr(), y.data_ptr(), n, b, -1, t.data_ptr(),
+                            w.data_ptr() if w is not None else None, sid) == 0
+    torch.cuda.synchronize()
+    streams = [torch.cuda.Stream(device=dev) for _ in shapes]
+    outs = [[torch.empty_like(x) for _ in range(8)] for x in xs]
+    # each thread's own workspace (the four-step chain writes it), allocated
+    # here and kept alive until every stream has finished
+    wks = [torch.empty_like(w) if w is not None else None for w in works]
+    errors = []
+
+    def worker(k):
+        n, b = shapes[k]
+        s = streams[k].cuda_stream
+        wk = wks[k]
+        for o in outs[k]:
+            rc = lib.bfly_c2c(xs[k].data_ptr(), o.data_ptr(), n, b, -1, tabs[k].data_ptr(),
+                              wk.data_ptr() if wk is not None else None, s)
+            if rc:
+                errors.append((k, rc))
+
+    threads = [threading.Thread(target=worker, args=(k,)) for k in range(len(shapes))]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join()
+    torch.cuda.synchronize()
+    assert not errors, errors
+    for k in range(len(shapes)):
+        for o in outs[k]:
+            assert torch.equal(o, alone[k]), shapes[k]
