@@ -148,11 +148,31 @@ __device__ __forceinline__ void dft16_mfma(float2 (&v)[16], const half8 (&ahi)[2
 // entry [k][q] of the pass table at tw[k PITCH + q]), runs the DFT and,
 // unless LAST, writes the outputs to `lds` in Stockham order.  The caller
 // synchronises.
-template <int N, int Ns, int DIR, bool LAST, int PITCH = 16>
+template <int N, int Ns, int DIR, bool LAST, int PITCH = 16, bool TWO_LEVEL = false>
 __device__ __forceinline__ void pass16_mfma(float2 (&v)[16], float2* lds, const float2* __restrict__ tw,
-                                            int w, int lane, const half8 (&ahi)[2], const half8 (&alo)[2]) {
+                                            int w, int lane, const half8 (&ahi)[2], const half8 (&alo)[2],
+                                            const float2* ab = nullptr) {
     const int h = lane >> 4, c = lane & 15;
-    if constexpr (Ns > 1) {
+    if constexpr (TWO_LEVEL) {
+        // W_N^{q k} for the lane's q = 4h .. 4h+3 from the two 64-entry LDS
+        // tables ab = [W_N^j, j < 64 | W_N^{64 j}]: the first by one product,
+        // the next three by successive multiplication with W_N^k
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            const int j = 64 * w + 16 * i + c;
+            const int k = j & (Ns - 1);
+            const int m0 = 4 * h * k;  // < 16 Ns = N
+            const float2 a0 = ab[m0 & 63], b0 = ab[64 + (m0 >> 6)];
+            const float2 a1 = ab[k & 63], b1 = ab[64 + (k >> 6)];
+            float2 wq = make_float2(a0.x * b0.x - a0.y * b0.y, a0.x * b0.y + a0.y * b0.x);
+            const float2 ws = make_float2(a1.x * b1.x - a1.y * b1.y, a1.x * b1.y + a1.y * b1.x);
+#pragma unroll
+            for (int e = 0; e < 4; e++) {
+                v[4 * i + e] = cmul_tw<DIR>(v[4 * i + e], wq);
+                wq = make_float2(wq.x * ws.x - wq.y * ws.y, wq.x * ws.y + wq.y * ws.x);
+            }
+        }
+    } else if constexpr (Ns > 1) {
 #pragma unroll
         for (int i = 0; i < 4; i++) {
             const int j = 64 * w + 16 * i + c;
@@ -226,8 +246,9 @@ __device__ __forceinline__ void load16_global(float2 (&v)[16], const float2* __r
 template <int I, int DIR, int PITCH1 = 16>
 __device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int* s_exp,
                                                const float2* __restrict__ tw, const float2* tw1,
-                                               const half8 (&ahi)[2], const half8 (&alo)[2], int t,
-                                               float2* __restrict__ dst, bool active) {
+                                               const float2* ab2, const half8 (&ahi)[2],
+                                               const half8 (&alo)[2], int t, float2* __restrict__ dst,
+                                               bool active) {
     using PL = PlanOf<I>;
     constexpr int N = PL::N, T = PL::T;
     constexpr int R2 = bfly_plan::kPlans[I].rad[2];
@@ -262,7 +283,7 @@ __device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int
     if constexpr (R2 == 16) {
         read16_lds<N>(v, lds, w, lane);
         __syncthreads();
-        pass16_mfma<N, 256, DIR, true>(v, lds, tw2, w, lane, ahi, alo);
+        pass16_mfma<N, 256, DIR, true, 16, true>(v, lds, tw2, w, lane, ahi, alo, ab2);
         if (!active) return;
 #pragma unroll
         for (int i = 0; i < 4; i++)
@@ -285,7 +306,12 @@ __device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int
         for (int m = 0; m < M; m++) {
             const int k = (t + m * T) & (Ns - 1);
 #pragma unroll
-            for (int r = 1; r < R2; r++) v[m * R2 + r] = cmul_tw<DIR>(v[m * R2 + r], tw2[k * R2 + r]);
+            for (int r = 1; r < R2; r++) {
+                // W_N^{r k} = A[r k mod 64] B[r k / 64] (r k < N)
+                const int e = r * k;
+                const float2 a = ab2[e & 63], b = ab2[64 + (e >> 6)];
+                v[m * R2 + r] = cmul_tw<DIR>(v[m * R2 + r], make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x));
+            }
             dft<R2, DIR>(&v[m * R2]);
         }
         if (!active) return;
@@ -298,6 +324,17 @@ __device__ __forceinline__ void mfma_transform(float2 (&v)[16], float2* lds, int
 }
 
 // One-shot grid: B = 256/T transforms per workgroup.
+// The last pass's twiddles come from the two tables [W_N^j, j < 64 |
+// W_N^{64 j}, j < N/64] at the end of the local table, copied into LDS once
+// per workgroup (64 + N/64 <= 128 entries).
+template <int I>
+__device__ __forceinline__ void fill_ab2(float2* ab2, const float2* __restrict__ tw, int tid) {
+    constexpr int LEN = 64 + PlanOf<I>::N / 64;
+    static_assert(LEN <= 128, "two tables of at most 64 entries");
+    const float2* ab = tw + bfly_plan::stage_twiddle_len(I);
+    for (int i = tid; i < LEN; i += bfly_plan::kBlock) ab2[i] = ab[i];
+}
+
 template <int I, int DIR>
 __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __restrict__ in,
                                                               float2* __restrict__ out,
@@ -306,7 +343,10 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
     constexpr int N = PlanOf<I>::N, T = PlanOf<I>::T, B = bfly_plan::kBlock / T, WAVES = T / 64;
     __shared__ float2 lds_all[B * lds_len(N)];
     __shared__ int s_exp[B * WAVES];
+    __shared__ float2 ab2[128];
     const int tid = threadIdx.x;
+    fill_ab2<I>(ab2, tw, tid);
+    // (the transform's first LDS barrier orders these writes before the reads)
     const int bl = tid / T, t = tid % T;
     const uint64_t tr = (uint64_t)blockIdx.x * B + bl;
     const bool active = tr < batch;
@@ -314,8 +354,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma(const float2* __
     dft16_operands<DIR>(t & 63, ahi, alo);
     float2 v[16];
     load16_global<N>(v, in + tr * N, active, t >> 6, t & 63);
-    mfma_transform<I, DIR>(v, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, tw, ahi, alo, t, out + tr * N,
-                           active);
+    mfma_transform<I, DIR>(v, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, tw, ab2, ahi, alo, t,
+                           out + tr * N, active);
 }
 
 // Streaming form: a resident grid walks the batch and loads each
@@ -332,9 +372,11 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma_stream(const flo
     __shared__ float2 lds_all[B * lds_len(N)];
     __shared__ int s_exp[B * WAVES];
     __shared__ float4 tw1s4[16 * PITCH1 / 2];
+    __shared__ float2 ab2[128];
     float2* tw1s = reinterpret_cast<float2*>(tw1s4);
     const int tid = threadIdx.x;
     for (int i = tid; i < 256; i += bfly_plan::kBlock) tw1s[(i / 16) * PITCH1 + i % 16] = tw[i];
+    fill_ab2<I>(ab2, tw, tid);
     // (the transform's first LDS barrier orders these writes before the reads)
     const int bl = tid / T, t0 = tid % T;
     const uint32_t groups = (uint32_t)((batch + B - 1) / B);
@@ -355,8 +397,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2c_mfma_stream(const flo
             load16_global<N>(nxt, in + trn * N, trn < batch, t >> 6, t & 63);
         }
         const uint64_t tr = (uint64_t)g * B + bl;
-        mfma_transform<I, DIR, PITCH1>(cur, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, tw1s, ahi, alo,
-                                       t, out + tr * N, tr < batch);
+        mfma_transform<I, DIR, PITCH1>(cur, lds_all + bl * lds_len(N), s_exp + bl * WAVES, tw, tw1s, ab2, ahi,
+                                       alo, t, out + tr * N, tr < batch);
 #pragma unroll
         for (int i = 0; i < 16; i++) cur[i] = nxt[i];
     }
