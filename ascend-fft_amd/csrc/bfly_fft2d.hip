@@ -3,18 +3,19 @@
 //
 // X[k1][k2] = sum_{j1, j2} x[j1][j2] W_n1^{j1 k1} W_n2^{j2 k2} is separable:
 // the row pass is the C2C kernels over batch * n1 rows of n2 points; the
-// column pass runs k_cols, which takes C = 256 / T adjacent columns per
-// workgroup (T = threads per column transform), so every load and store is
-// a run of C consecutive complex values (C * 8 bytes: 128 B at 256-point
-// columns, 512 B at 64).  Column lengths up to 256 are one k_cols pass;
-// 512..4096 are split n1 = P * Q (four-step along the columns):
+// column pass runs k_cols, which takes C = WG / T adjacent columns per
+// workgroup of WG threads (T = threads per column transform), so every load
+// and store is a run of C consecutive complex values (C * 8 bytes: 128 B at
+// 256-point columns, 512 B at 64).  Column lengths up to 2048 are one
+// k_cols pass (512 / 1024 / 2048 in 512 / 1024 / 1024-thread workgroups);
+// 4096 is split n1 = P * Q = 64 * 64 (four-step along the columns):
 //
 //   j1 = Q p + q,  k1 = kp + P kq
 //   Y[Q kp + q][c] = W_n1^{q kp} * FFT_P over p of x[Q p + q][c]   (k_cols, twiddle)
 //   X[kp + P kq][c] = FFT_Q over q of Y[Q kp + q][c]              (k_cols, row map)
 //
-// so the whole transform is two HBM round trips up to n1 = 256 and three
-// above.  The column transforms use the LDS twiddle tables of the streaming
+// so the whole transform is two HBM round trips up to n1 = 2048 and three
+// at 4096.  The column transforms use the LDS twiddle tables of the streaming
 // C2C kernels (stockham_lt); the inter-pass twiddle is W_n1^e =
 // A[e mod 64] * B[e / 64] from two LDS tables.
 #include <hip/hip_runtime.h>
@@ -64,7 +65,12 @@ __global__ __launch_bounds__(WG) void k_cols(const float2* __restrict__ in, floa
     if constexpr (TW)
         for (int i = tid; i < 128; i += WG) wt[i] = wab[i];
     const int c = tid % C, t = tid / C;
-    const uint32_t blk = blockIdx.x;
+    // groups narrower than a 128-byte line (C < 16) share lines: consecutive
+    // groups then run on one XCD (workgroups go to the 8 XCDs in turn), so
+    // the shared lines meet in its L2
+    const uint32_t nb = gridDim.x;
+    const uint32_t blk =
+        (C < 16 && nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
     const uint32_t g = blk % cv.groups;
     const uint64_t mi = blk / cv.groups;
     const uint64_t b = mi / cv.mats, m = mi % cv.mats;
@@ -99,19 +105,17 @@ __global__ __launch_bounds__(WG) void k_cols(const float2* __restrict__ in, floa
         }
 }
 
-template <int I, bool TW>
+template <int I, bool TW, int WG = bfly_plan::kBlock>
 hipError_t launch_cols(const float2* in, float2* out, const float2* tw, const float2* wab, ColView cv,
                        uint64_t batch, int sign, hipStream_t s) {
-    constexpr int C = bfly_plan::kBlock / PlanOf<I>::T;
+    constexpr int C = WG / PlanOf<I>::T;
     cv.groups = (cv.ncols + C - 1) / C;
     const uint64_t blocks = batch * cv.mats * cv.groups;
     if (blocks == 0 || blocks > 0x7fffffffull) return hipErrorInvalidValue;
     if (sign < 0)
-        hipLaunchKernelGGL((k_cols<I, -1, TW>), dim3((unsigned)blocks), dim3(bfly_plan::kBlock), 0, s, in, out, tw,
-                           wab, cv);
+        hipLaunchKernelGGL((k_cols<I, -1, TW, WG>), dim3((unsigned)blocks), dim3(WG), 0, s, in, out, tw, wab, cv);
     else
-        hipLaunchKernelGGL((k_cols<I, 1, TW>), dim3((unsigned)blocks), dim3(bfly_plan::kBlock), 0, s, in, out, tw,
-                           wab, cv);
+        hipLaunchKernelGGL((k_cols<I, 1, TW, WG>), dim3((unsigned)blocks), dim3(WG), 0, s, in, out, tw, wab, cv);
     return hipGetLastError();
 }
 
@@ -127,8 +131,17 @@ hipError_t cols(uint32_t len, const float2* in, float2* out, const float2* tw, c
         case 64: return launch_cols<idx(64), TW>(in, out, tw, wab, cv, batch, sign, s);
         case 128: return launch_cols<idx(128), TW>(in, out, tw, wab, cv, batch, sign, s);
         case 256: return launch_cols<idx(256), TW>(in, out, tw, wab, cv, batch, sign, s);
-        default: return hipErrorInvalidValue;
+        default: break;
     }
+    // 512 / 1024 / 2048-point columns in one pass: 16 / 16 / 8 columns per
+    // workgroup in 512 / 1024 / 1024-thread workgroups (70 / 139 / 139 KiB
+    // of LDS)
+    if constexpr (!TW) {
+        if (len == 512) return launch_cols<idx(512), TW, 512>(in, out, tw, wab, cv, batch, sign, s);
+        if (len == 1024) return launch_cols<idx(1024), TW, 1024>(in, out, tw, wab, cv, batch, sign, s);
+        if (len == 2048) return launch_cols<idx(2048), TW, 1024>(in, out, tw, wab, cv, batch, sign, s);
+    }
+    return hipErrorInvalidValue;
 }
 
 }  // namespace

@@ -96,11 +96,11 @@ bool fft2d_ok(uint32_t n1, uint32_t n2) {
 Fft2dLayout fft2d_layout(uint32_t n1) {
     Fft2dLayout t{};
     if (!pow2(n1) || n1 < 16u || n1 > (uint32_t)bfly_plan::kMaxLocalN) return t;
-    if (n1 <= 256u) {
+    if (n1 <= 2048u) {
         t.p = n1;
         t.q = 1;
     } else {
-        t.p = n1 >= 2048u ? 64u : 32u;
+        t.p = 64;
         t.q = n1 / t.p;
     }
     t.p_table = 0;

@@ -50,8 +50,7 @@ bool needs_work(int kind, uint32_t n, uint32_t batch, int variant);
 bool split_mfma_ok(uint32_t n);
 
 // 2-D C2C of [batch][n1][n2]: n1, n2 powers of two in [16, 4096].  Column
-// lengths up to 256 run as one pass (p = n1, q = 1); above, n1 = p * q
-// (p = 32 / 32 / 64 / 64 for 512 / 1024 / 2048 / 4096).  The column table
+// lengths up to 2048 run as one pass (p = n1, q = 1); 4096 = p * q = 64 * 64.  The column table
 // of n1 is [local C2C table of p][local C2C table of q][W_n1^j, j < 64 |
 // W_n1^{64 j}, j < n1 / 64, zero-padded to 128 entries] (complex offsets).
 struct Fft2dLayout {

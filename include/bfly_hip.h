@@ -163,12 +163,12 @@ BFLY_API int bfly_c2c_inv(const float* in, float* out, uint32_t n, uint32_t batc
  * (config/test_matrix.json:94-99).  `table_rows` is the device copy of
  * bfly_table_fill(BFLY_KIND_C2C, n2), `table_cols` of bfly_fft2d_table_fill(n1);
  * `work` holds bfly_fft2d_workspace_bytes(n1, n2, batch) bytes.  Two HBM
- * round trips up to n1 = 256, three above. */
+ * round trips up to n1 = 2048, three at 4096. */
 /* Strided 1-D C2C along the first axis of `batch` row-major [n][ncols]
  * arrays (element stride ncols, adjacent transforms): X[k][c] = sum_j
  * x[j][c] e^{sign 2 pi i j k / n}; n a power of two in [16, 4096], any
  * ncols >= 1.  The column pass of bfly_fft2d on its own: one HBM round trip
- * up to n = 256, two above through `work`
+ * up to n = 2048, two at 4096 through `work`
  * (bfly_c2c_columns_workspace_bytes; NULL allowed when that is 0).
  * `table_cols` is the device copy of bfly_fft2d_table_fill(n).  No
  * counterpart in the reference (non-contiguous layouts are future work

@@ -221,7 +221,7 @@ def test_fft2d_column_table_layout(n1):
     tab = np.empty(nf, np.float32)
     assert s.bfly_fft2d_table_fill(n1, tab.ctypes.data) == 0
     c = tab.view(np.complex64)
-    p = n1 if n1 <= 256 else (64 if n1 >= 2048 else 32)
+    p = n1 if n1 <= 2048 else 64
     q = n1 // p
     # [local C2C table of p][local C2C table of q][W_n1^j, j<64 | W_n1^{64 j}] (128 entries)
     lp = s.bfly_table_floats(bfly.KIND_C2C, p) // 2
@@ -267,17 +267,17 @@ def test_strided_columns_guards_through_stub():
     buf = np.zeros(16, np.float32)
     p = buf.ctypes.data
     i, o, w, t = p, p + (1 << 30), p + (2 << 30), p + (3 << 30)
-    assert s.bfly_c2c_columns(i, o, 256, 100, 3, -1, t, None, None) == 0      # one pass: no workspace
-    assert s.bfly_c2c_columns_workspace_bytes(256, 100, 3) == 0
-    assert s.bfly_c2c_columns_workspace_bytes(512, 100, 3) == 8 * 512 * 100 * 3
-    assert s.bfly_c2c_columns(i, o, 512, 100, 3, -1, t, None, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c_columns(i, o, 512, 100, 3, 1, t, w, None) == 0
+    assert s.bfly_c2c_columns(i, o, 1024, 100, 3, -1, t, None, None) == 0     # one pass: no workspace
+    assert s.bfly_c2c_columns_workspace_bytes(1024, 100, 3) == 0
+    assert s.bfly_c2c_columns_workspace_bytes(4096, 100, 3) == 8 * 4096 * 100 * 3
+    assert s.bfly_c2c_columns(i, o, 4096, 100, 3, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_columns(i, o, 4096, 100, 3, 1, t, w, None) == 0
     for n, ncols, b in ((8, 4, 1), (8192, 4, 1), (96, 4, 1), (64, 0, 1), (64, 4, 0)):
         assert s.bfly_c2c_columns(i, o, n, ncols, b, -1, t, w, None) == bfly.ERR_SHAPE, (n, ncols, b)
     assert s.bfly_c2c_columns(i, o, 64, 4, 1, 2, t, w, None) == bfly.ERR_SHAPE
     assert s.bfly_c2c_columns(i, o, 64, 4, 1, -1, None, w, None) == _lib.ERR_TABLE
     assert s.bfly_c2c_columns(i, i + 8, 64, 4, 1, -1, t, w, None) == bfly.ERR_SHAPE       # overlap
-    assert s.bfly_c2c_columns(i, o, 1024, 4, 1, -1, t, o + 64, None) == bfly.ERR_SHAPE    # work inside out
+    assert s.bfly_c2c_columns(i, o, 4096, 4, 1, -1, t, o + 64, None) == bfly.ERR_SHAPE    # work inside out
 
 
 def test_python_modules_import_and_variant_ids_match_header():
