@@ -65,12 +65,12 @@ __global__ __launch_bounds__(WG) void k_cols(const float2* __restrict__ in, floa
     if constexpr (TW)
         for (int i = tid; i < 128; i += WG) wt[i] = wab[i];
     const int c = tid % C, t = tid / C;
-    // groups narrower than a 128-byte line (C < 16) share lines: consecutive
-    // groups then run on one XCD (workgroups go to the 8 XCDs in turn), so
-    // the shared lines meet in its L2
-    const uint32_t nb = gridDim.x;
-    const uint32_t blk =
-        (C < 16 && nb % 8 == 0) ? (blockIdx.x % 8) * (nb / 8) + blockIdx.x / 8 : blockIdx.x;
+    // groups narrower than a 128-byte line (C = 8) share each line with the
+    // next group: workgroups are dealt to the 8 XCDs in turn, so within each
+    // window of 16 the pair of groups 2x, 2x+1 goes to XCD x (dealt 8 apart)
+    // and meets in its L2, while the 8 XCDs still work on adjacent columns
+    const uint32_t nb = gridDim.x, b0 = blockIdx.x;
+    const uint32_t blk = (C < 16 && (b0 | 15) < nb) ? (b0 & ~15u) | ((b0 & 7u) << 1) | ((b0 >> 3) & 1u) : b0;
     const uint32_t g = blk % cv.groups;
     const uint64_t mi = blk / cv.groups;
     const uint64_t b = mi / cv.mats, m = mi % cv.mats;
