@@ -208,23 +208,25 @@ def case_timeout_s(reps):
 
 
 def run_with_rss(argv, env, timeout_s):
+    """Run argv sampling its RSS; the output goes to a temporary file (a pipe
+    read only at the end could fill and stall the child)."""
+    import tempfile
     t0 = time.perf_counter()
-    proc = subprocess.Popen(argv, env=env, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True,
-                            errors="replace")
-    samples, timed_out = [], False
-    while proc.poll() is None:
-        if time.perf_counter() - t0 > timeout_s:
-            timed_out = True
-            proc.kill()
-            break
-        rss = read_rss_bytes(proc.pid)
-        if rss is not None:
-            samples.append((time.perf_counter() - t0, rss))
-        time.sleep(RSS_SAMPLE_S)
-    out = proc.stdout.read() if proc.stdout else ""
-    if proc.stdout:
-        proc.stdout.close()
-    proc.wait()
+    with tempfile.TemporaryFile(mode="w+", errors="replace") as log:
+        proc = subprocess.Popen(argv, env=env, stdout=log, stderr=subprocess.STDOUT)
+        samples, timed_out = [], False
+        while proc.poll() is None:
+            if time.perf_counter() - t0 > timeout_s:
+                timed_out = True
+                proc.kill()
+                break
+            rss = read_rss_bytes(proc.pid)
+            if rss is not None:
+                samples.append((time.perf_counter() - t0, rss))
+            time.sleep(RSS_SAMPLE_S)
+        proc.wait()
+        log.seek(0)
+        out = log.read()
     return proc.returncode, out, samples, timed_out, (time.perf_counter() - t0) * 1e3
 
 
