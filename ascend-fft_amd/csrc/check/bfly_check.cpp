@@ -249,6 +249,11 @@ int main(int argc, char** argv) {
             return 2;
         }
     }
+    if (kind != BFLY_KIND_C2C && variant != BFLY_VARIANT_AUTO && variant != BFLY_VARIANT_PLAIN &&
+        variant != BFLY_VARIANT_STREAM && variant != BFLY_VARIANT_STREAM_LT) {
+        std::printf("AB_VARIANT %d does not apply to %s (auto, plain, stream, stream_lt)\n", variant, dir);
+        return 2;
+    }
     const int launches = kind == BFLY_KIND_C2C ? bfly_c2c_launches(n, batch, variant) : 1;
     if (launches < 0) {
         std::printf("AB_VARIANT %d does not apply to c2c n=%u batch=%u\n", variant, n, batch);

@@ -49,6 +49,7 @@ def run(exe, *args, **envs):
     ((8192, 2), {"AB_LONG_BOUNDARY_IMPL": "separate"}, "AB_LONG_BOUNDARY_IMPL must be fused"),
     ((256, 2), {"AB_DIR": "r2c", "AB_SGN": "+1"}, "fixed sign"),
     ((256, 2), {"AB_VARIANT": "fastest"}, "unknown AB_VARIANT"),
+    ((256, 2), {"AB_DIR": "r2c", "AB_VARIANT": "stream2"}, "does not apply to r2c"),
     ((256, 2), {"AB_INPUT_FILE": "/nonexistent/in.bin"}, "AB_INPUT_FILE read failed"),
 ])
 def test_bad_arguments_exit_2(cli, args, envs, msg):
