@@ -65,7 +65,7 @@ SYMBOLS = (
     "bfly_c2c_launches", "bfly_c2c_traced", "bfly_c2c_columns_workspace_bytes", "bfly_c2c_columns",
     "bfly_fft2d_table_floats", "bfly_fft2d_table_fill",
     "bfly_fft2d_workspace_bytes", "bfly_fft2d", "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
-    "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_fwd", "bfly_c2c_inv", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
+    "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_fwd", "bfly_c2c_inv", "bfly_c2c_scaled", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
     "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
     "bfly_group_query", "bfly_group_destroy", "bfly_mode_mix", "bfly_probe_copy", "bfly_probe_device",
 )
@@ -131,6 +131,7 @@ def _bind(lib):
         "bfly_c2c_ex": (i32, [vp, vp, u32, u32, i32, i32, vp, vp, vp]),
         "bfly_c2c_fwd": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_inv": (i32, [vp, vp, u32, u32, vp, vp, vp]),
+        "bfly_c2c_scaled": (i32, [vp, vp, u32, u32, i32, C.c_float, vp, vp, vp]),
         "bfly_rt_teardown": (i32, []),
         "bfly_r2c": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),

@@ -8,6 +8,8 @@ tensor's device.  Layouts are the reference's transform contract:
 """
 from __future__ import annotations
 
+import math
+
 import numpy as np
 import torch
 
@@ -56,18 +58,42 @@ def _check_device(t: torch.Tensor, name: str) -> None:
 
 
 def c2c_raw(x: torch.Tensor, out: torch.Tensor, n: int, batch: int, sign: int = -1,
-            work: torch.Tensor | None = None) -> None:
-    """bfly_c2c on interleaved float32 buffers (no allocation except tables)."""
+            work: torch.Tensor | None = None, scale: float | None = None) -> None:
+    """bfly_c2c on interleaved float32 buffers (no allocation except tables);
+    bfly_c2c_scaled when `scale` is given."""
     tab = table(_lib.KIND_C2C, n, x.device)
     if work is None and _lib.hip().bfly_workspace_bytes(_lib.KIND_C2C, n, batch):
         work = workspace(_lib.KIND_C2C, n, batch, x.device)
-    rc = _lib.hip().bfly_c2c(x.data_ptr(), out.data_ptr(), n, batch, sign, tab.data_ptr(),
-                             work.data_ptr() if work is not None else None, _stream(x.device))
-    _lib.check(rc, f"bfly_c2c(n={n}, batch={batch}, sign={sign})")
+    wp = work.data_ptr() if work is not None else None
+    if scale is None:
+        rc = _lib.hip().bfly_c2c(x.data_ptr(), out.data_ptr(), n, batch, sign, tab.data_ptr(), wp,
+                                 _stream(x.device))
+        _lib.check(rc, f"bfly_c2c(n={n}, batch={batch}, sign={sign})")
+    else:
+        rc = _lib.hip().bfly_c2c_scaled(x.data_ptr(), out.data_ptr(), n, batch, sign, scale, tab.data_ptr(), wp,
+                                        _stream(x.device))
+        _lib.check(rc, f"bfly_c2c_scaled(n={n}, batch={batch}, sign={sign}, scale={scale})")
 
 
-def c2c(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
-    """Unnormalised C2C along the last axis (sign -1 forward, +1 inverse)."""
+def norm_scale(n: int, sign: int, norm: str | None) -> float | None:
+    """Output scale of numpy's `norm` conventions for a length-n C2C: None
+    (the reference's unnormalised transform both ways), "backward" (1/n on
+    the inverse), "ortho" (1/sqrt(n) both ways), "forward" (1/n on the
+    forward).  None means no scaling."""
+    if norm is None:
+        return None
+    if norm not in ("backward", "ortho", "forward"):
+        raise ValueError(f"norm must be None, 'backward', 'ortho' or 'forward', not {norm!r}")
+    if norm == "ortho":
+        return 1.0 / math.sqrt(n)
+    inverse = sign > 0
+    return 1.0 / n if (norm == "backward") == inverse else None
+
+
+def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None) -> torch.Tensor:
+    """C2C along the last axis (sign -1 forward, +1 inverse); unnormalised
+    unless `norm` names numpy's convention, whose scale the transform's last
+    store applies."""
     _check_device(x, "x")
     if x.dtype == torch.complex64:
         n = x.shape[-1]
@@ -84,7 +110,7 @@ def c2c(x: torch.Tensor, sign: int = -1) -> torch.Tensor:
     if batch == 0:
         return out
     of = torch.view_as_real(out) if x.dtype == torch.complex64 else out
-    c2c_raw(xf, of, n, batch, sign)
+    c2c_raw(xf, of, n, batch, sign, scale=norm_scale(n, sign, norm))
     return out
 
 

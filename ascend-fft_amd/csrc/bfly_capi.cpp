@@ -2,6 +2,7 @@
 // on the HIP runtime: argument guards, runtime wrappers and kernel dispatch.
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
@@ -229,8 +230,8 @@ int bfly_c2c_traced(const float* in, float* out, uint32_t n, uint32_t batch, int
     return hipEventRecord(ev(launches), S(stream)) == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH;
 }
 
-int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
-                const float* table, void* work, void* stream) {
+static int c2c_checked(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
+                       const float* table, void* work, void* stream, float scale) {
     if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, work, variant))
         return g;
     if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
@@ -238,7 +239,20 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
     const bool uses_work = bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant);
     if (!bfly_host::disjoint(BFLY_KIND_C2C, in, out, uses_work ? work : nullptr, n, batch, 0, 0))
         return BFLY_ERR_SHAPE;
-    return launched(bfly_kern::c2c(in, out, n, batch, sign, variant, table, work, S(stream)));
+    return launched(bfly_kern::c2c(in, out, n, batch, sign, variant, table, work, S(stream), nullptr, scale));
+}
+
+int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
+                const float* table, void* work, void* stream) {
+    return c2c_checked(in, out, n, batch, sign, variant, table, work, stream, 1.0f);
+}
+
+int bfly_c2c_scaled(const float* in, float* out, uint32_t n, uint32_t batch, int sign, float scale,
+                    const float* table, void* work, void* stream) {
+    if (!std::isfinite(scale)) return BFLY_ERR_SHAPE;
+    // validate n before it picks a variant; check_execute repeats it
+    if (!bfly_supported(BFLY_KIND_C2C, n)) return BFLY_ERR_SHAPE;
+    return c2c_checked(in, out, n, batch, sign, bfly_host::scaled_variant(n, batch), table, work, stream, scale);
 }
 
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,

@@ -169,7 +169,7 @@ constexpr int lt_wg() { return PlanOf<I>::T > bfly_plan::kBlock ? PlanOf<I>::T :
 template <int I, int DIR>
 __global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-    const float2* __restrict__ tw2g, uint32_t batch) {
+    const float2* __restrict__ tw2g, uint32_t batch, float scale) {
     using PL = PlanOf<I>;
     constexpr int WG = lt_wg<I>();
     constexpr int N = PL::N, T = PL::T, B = WG / T;
@@ -221,6 +221,10 @@ __global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
         run_plan_lt<I, T, DIR, LX_PAD>(cur, lds + bl * lds_len(N), 0, ltw, ltw + NLO, ltw + NLO + NHI, t,
                                        std::make_index_sequence<PL::NRAD>{});
         const uint64_t row = (uint64_t)g * B + bl;
+        if (scale != 1.0f) {  // uniform: the normalised entry point only
+#pragma unroll
+            for (int i = 0; i < P; i++) cur[i] = make_float2(cur[i].x * scale, cur[i].y * scale);
+        }
         if (row < batch) {
             float2* dst = out + row * N;
 #pragma unroll
@@ -453,7 +457,7 @@ __global__ __launch_bounds__(WG) void k_long_cols(
 
 template <int I2, int N1, int DIR, int WG = bfly_plan::kBlock>
 __global__ __launch_bounds__(WG) void k_long_rows(
-    const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2) {
+    const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2, float scale) {
     using PL = PlanOf<I2>;
     constexpr int N2 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -489,6 +493,10 @@ __global__ __launch_bounds__(WG) void k_long_rows(
     __syncthreads();
     run_plan_lt<I2, T, DIR, LX_PAD>(v, L, 0, t1s, t1s, t1s, t, std::make_index_sequence<2>{});
     float2* dst = out + b * N + k1_0 + c;
+    if (scale != 1.0f) {  // uniform: the normalised entry point only
+#pragma unroll
+        for (int i = 0; i < P; i++) v[i] = make_float2(v[i].x * scale, v[i].y * scale);
+    }
 #pragma unroll
     for (int m = 0; m < P / RL; m++)
 #pragma unroll
@@ -528,20 +536,20 @@ hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint3
 
 template <int I, int DIR>
 hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw, const float2* tw2g,
-                         uint32_t batch, hipStream_t s) {
+                         uint32_t batch, hipStream_t s, float scale = 1.0f) {
     constexpr int WG = lt_wg<I>(), B = WG / PlanOf<I>::T;
     const uint64_t groups = (batch + B - 1) / B;
     static bfly_kern::ResidentCache rc;
     const int cap = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>), WG);
     if (cap <= 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)cap));
-    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(WG), 0, s, in, out, tw, tw2g, batch);
+    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(WG), 0, s, in, out, tw, tw2g, batch, scale);
     return hipGetLastError();
 }
 
 template <int I>
 hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uint32_t batch,
-                            int sign, int variant, hipStream_t s) {
+                            int sign, int variant, hipStream_t s, float scale) {
     if (variant == BFLY_VARIANT_STREAM2) {
         constexpr int T = PlanOf<I>::T, B = bfly_plan::kBlock / T;
         const uint64_t groups = (batch + B - 1) / B;
@@ -559,8 +567,8 @@ hipError_t launch_c2c_local(const float2* in, float2* out, const float2* tw, uin
     }
     if (variant == BFLY_VARIANT_STREAM_LT) {
         const float2* tw2g = tw + bfly_host::table_layout(0, PlanOf<I>::N).extra;
-        return sign < 0 ? launch_c2c_lt<I, -1>(in, out, tw, tw2g, batch, s)
-                        : launch_c2c_lt<I, 1>(in, out, tw, tw2g, batch, s);
+        return sign < 0 ? launch_c2c_lt<I, -1>(in, out, tw, tw2g, batch, s, scale)
+                        : launch_c2c_lt<I, 1>(in, out, tw, tw2g, batch, s, scale);
     }
     const bool stream = variant == BFLY_VARIANT_STREAM || variant == BFLY_VARIANT_STREAM_SWZ;
     const bool swz = variant == BFLY_VARIANT_PLAIN_SWZ || variant == BFLY_VARIANT_STREAM_SWZ;
@@ -635,7 +643,7 @@ hipError_t launch_c2c_single(const float2* in, float2* out, const float2* tw, ui
 template <int I1, int I2, int DIR>
 hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const float2* tw1,
                            const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s,
-                           hipEvent_t mid) {
+                           hipEvent_t mid, float scale) {
     constexpr int N1 = kPlans[I1].n, N2 = kPlans[I2].n;
     // Column kernel in 512-thread workgroups when that still fills every
     // resident slot: twice the columns per workgroup, 256-byte runs (65536 x
@@ -657,16 +665,16 @@ hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const floa
     if (e != hipSuccess) return e;
     if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),
-                       0, s, ws, out, tw2);
+                       0, s, ws, out, tw2, scale);
     return hipGetLastError();
 }
 
 template <int I1, int I2>
 hipError_t launch_long(const float2* in, float2* out, float2* ws, const float2* tw1,
                        const float2* tw2, const float2* wN, uint32_t batch, int sign,
-                       hipStream_t s, hipEvent_t mid) {
-    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s, mid)
-                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s, mid);
+                       hipStream_t s, hipEvent_t mid, float scale) {
+    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s, mid, scale)
+                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s, mid, scale);
 }
 
 constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
@@ -693,22 +701,22 @@ int resident(ResidentCache& cache, const void* kernel, int block) {
 }
 
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
-               const float* table, void* work, hipStream_t s, hipEvent_t mid) {
+               const float* table, void* work, hipStream_t s, hipEvent_t mid, float scale) {
     if (variant == BFLY_VARIANT_SPLIT_MFMA || variant == BFLY_VARIANT_SPLIT_MFMA_STREAM)
         return c2c_mfma(in, out, n, batch, sign, variant == BFLY_VARIANT_SPLIT_MFMA_STREAM, table, s);
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     const float2* tw = reinterpret_cast<const float2*>(table);
     switch (n) {
-        case 16: return launch_c2c_local<idx(16)>(i2, o2, tw, batch, sign, variant, s);
-        case 32: return launch_c2c_local<idx(32)>(i2, o2, tw, batch, sign, variant, s);
-        case 64: return launch_c2c_local<idx(64)>(i2, o2, tw, batch, sign, variant, s);
-        case 128: return launch_c2c_local<idx(128)>(i2, o2, tw, batch, sign, variant, s);
-        case 256: return launch_c2c_local<idx(256)>(i2, o2, tw, batch, sign, variant, s);
-        case 512: return launch_c2c_local<idx(512)>(i2, o2, tw, batch, sign, variant, s);
-        case 1024: return launch_c2c_local<idx(1024)>(i2, o2, tw, batch, sign, variant, s);
-        case 2048: return launch_c2c_local<idx(2048)>(i2, o2, tw, batch, sign, variant, s);
-        case 4096: return launch_c2c_local<idx(4096)>(i2, o2, tw, batch, sign, variant, s);
+        case 16: return launch_c2c_local<idx(16)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 32: return launch_c2c_local<idx(32)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 64: return launch_c2c_local<idx(64)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 128: return launch_c2c_local<idx(128)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 256: return launch_c2c_local<idx(256)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 512: return launch_c2c_local<idx(512)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 1024: return launch_c2c_local<idx(1024)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 2048: return launch_c2c_local<idx(2048)>(i2, o2, tw, batch, sign, variant, s, scale);
+        case 4096: return launch_c2c_local<idx(4096)>(i2, o2, tw, batch, sign, variant, s, scale);
         default: break;
     }
     const bfly_host::TableLayout L = bfly_host::table_layout(0, n);
@@ -716,11 +724,11 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
         const float2 *ts = tw + L.single, *wN = tw + L.extra;
         switch (n) {
             case 8192:
-                return sign < 0 ? launch_c2c_lt<idx(8192), -1>(i2, o2, ts, wN, batch, s)
-                                : launch_c2c_lt<idx(8192), 1>(i2, o2, ts, wN, batch, s);
+                return sign < 0 ? launch_c2c_lt<idx(8192), -1>(i2, o2, ts, wN, batch, s, scale)
+                                : launch_c2c_lt<idx(8192), 1>(i2, o2, ts, wN, batch, s, scale);
             case 16384:
-                return sign < 0 ? launch_c2c_lt<idx(16384), -1>(i2, o2, ts, wN, batch, s)
-                                : launch_c2c_lt<idx(16384), 1>(i2, o2, ts, wN, batch, s);
+                return sign < 0 ? launch_c2c_lt<idx(16384), -1>(i2, o2, ts, wN, batch, s, scale)
+                                : launch_c2c_lt<idx(16384), 1>(i2, o2, ts, wN, batch, s, scale);
             default: break;
         }
     }
@@ -734,10 +742,10 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
     const float2 *t1 = tw + L.stage0, *t2 = tw + L.stage1, *wN = tw + L.extra;
     float2* ws = reinterpret_cast<float2*>(work);
     switch (n) {
-        case 8192: return launch_long<idx(64), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
-        case 16384: return launch_long<idx(128), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
-        case 32768: return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
-        case 65536: return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid);
+        case 8192: return launch_long<idx(64), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
+        case 16384: return launch_long<idx(128), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
+        case 32768: return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
+        case 65536: return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
         default: return hipErrorInvalidValue;
     }
 }

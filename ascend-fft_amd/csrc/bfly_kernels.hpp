@@ -19,9 +19,12 @@ struct ResidentCache {
 int resident(ResidentCache& cache, const void* kernel, int block);
 
 // `mid`, when given, is recorded on `s` between the two launches of the
-// four-step chain (unused by the one-launch paths).
+// four-step chain (unused by the one-launch paths).  `scale` multiplies
+// every output in the final store; the LDS-twiddle streaming kernels
+// (BFLY_VARIANT_STREAM_LT, and 8192 / 16384 under AUTO) and the four-step
+// chain apply it, the other variants require 1.
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
-               const float* table, void* work, hipStream_t s, hipEvent_t mid = nullptr);
+               const float* table, void* work, hipStream_t s, hipEvent_t mid = nullptr, float scale = 1.0f);
 // variant: BFLY_VARIANT_PLAIN (one-shot grid), BFLY_VARIANT_STREAM or
 // BFLY_VARIANT_STREAM_LT; `adjoint` runs the transform's adjoint (R2C: the
 // C2R's adjoint weighting of the bins; C2R: the R2C's).

@@ -4,6 +4,7 @@
 // without computing anything.  Same role as the reference's fake AscendCL
 // (tests/stubs/acl/acl_rt.h:15-53 of the reference): it lets the C++
 // Context/Plan guard and cache-invalidation tests run without a GPU.
+#include <cmath>
 #include <cstdint>
 #include <cstdio>
 #include <chrono>
@@ -199,6 +200,12 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream) {
     return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);
+}
+int bfly_c2c_scaled(const float* in, float* out, uint32_t n, uint32_t batch, int sign, float scale,
+                    const float* table, void* work, void* stream) {
+    if (!std::isfinite(scale)) return BFLY_ERR_SHAPE;
+    if (!bfly_supported(BFLY_KIND_C2C, n)) return BFLY_ERR_SHAPE;
+    return bfly_c2c_ex(in, out, n, batch, sign, bfly_host::scaled_variant(n, batch), table, work, stream);
 }
 int bfly_c2c_fwd(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
                  void* stream) {

@@ -256,6 +256,10 @@ int auto_variant(uint32_t n, uint32_t batch) {
     return BFLY_VARIANT_PLAIN;
 }
 
+int scaled_variant(uint32_t n, uint32_t batch) {
+    return n <= (uint32_t)bfly_plan::kMaxLocalN ? BFLY_VARIANT_STREAM_LT : auto_variant(n, batch);
+}
+
 // Real transforms, once the batch fills several rounds of resident
 // workgroups: streaming with LDS twiddles from n = 1024 (measured, MI355X:
 // R2C n=8192 b=4096 54.3 us vs 74.1 one-shot / 74.8 streaming; C2R n=4096

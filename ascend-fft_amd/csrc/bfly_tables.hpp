@@ -75,6 +75,9 @@ bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride);
 int auto_variant(uint32_t n, uint32_t batch);
 // The variant BFLY_VARIANT_AUTO resolves to for a real transform of (n, batch).
 int auto_real_variant(uint32_t n, uint32_t batch);
+// The variant bfly_c2c_scaled runs: one whose last store applies the scale
+// (the LDS-twiddle streaming kernel up to n = 8192, AUTO's choice above).
+int scaled_variant(uint32_t n, uint32_t batch);
 
 // Guards of bfly_mode_mix: op 0..2, non-null 8-byte aligned operands, every
 // size >= 1, an output overlapping neither input, a grid that fits.

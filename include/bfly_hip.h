@@ -153,6 +153,18 @@ BFLY_API int bfly_c2c_fwd(const float* in, float* out, uint32_t n, uint32_t batc
                           void* stream);
 BFLY_API int bfly_c2c_inv(const float* in, float* out, uint32_t n, uint32_t batch, const float* table, void* work,
                           void* stream);
+/* bfly_c2c with every output multiplied by `scale` in the transform's last
+ * store (no extra pass over HBM): scale 1/n on the inverse is the normalised
+ * inverse, 1/sqrt(n) both ways the unitary transform.  The reference has no
+ * counterpart (its C2C is unnormalised and only C2R applies 1/n,
+ * src/ascendc/fft_real.cpp:167-267); it serves callers of the public inverse
+ * who want numpy's norm="backward" / "ortho" / "forward" conventions.
+ * Runs the LDS-twiddle streaming kernels up to n = 8192 and AUTO's choice
+ * above, so the result equals fl(scale * bfly_c2c_ex(..., variant)) bit for
+ * bit for that variant.  `scale` must be finite (else BFLY_ERR_SHAPE);
+ * `work` as for bfly_c2c. */
+BFLY_API int bfly_c2c_scaled(const float* in, float* out, uint32_t n, uint32_t batch, int sign, float scale,
+                             const float* table, void* work, void* stream);
 
 /* ------------------------------------------------------------------ 2-D
  * 2-D C2C of `batch` row-major [n1][n2] arrays of interleaved complex

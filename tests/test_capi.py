@@ -180,6 +180,41 @@ def test_execute_guards_through_stub():
     assert cnt[3] == 11  # launches counted only for accepted calls
 
 
+def test_scaled_c2c_guards_through_stub():
+    s = bfly.stub()
+    buf = np.zeros(1 << 16, np.float32)
+    p = buf.ctypes.data
+    i, o, w, t = p, p + (1 << 30), p + (2 << 30), p + (3 << 30)
+    assert s.bfly_c2c_scaled(i, o, 4096, 3, 1, 1.0 / 4096, t, None, None) == 0
+    assert s.bfly_c2c_scaled(i, o, 16, 1, -1, -2.5, t, None, None) == 0
+    assert s.bfly_c2c_scaled(i, o, 4096, 3, 1, float("nan"), t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_scaled(i, o, 4096, 3, 1, float("inf"), t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_scaled(i, o, 4095, 3, 1, 1.0, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_scaled(i, o, 4096, 0, 1, 1.0, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_scaled(i, o, 4096, 3, 0, 1.0, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_scaled(i, o, 8192, 1, 1, 1.0, t, None, None) == 0  # LDS-twiddle kernel, no workspace
+    assert s.bfly_c2c_scaled(i, o, 16384, 64, 1, 1.0, t, None, None) == bfly.ERR_SHAPE  # four-step: work
+    assert s.bfly_c2c_scaled(i, o, 16384, 64, 1, 1.0, t, w, None) == 0
+    assert s.bfly_c2c_scaled(i, o, 16384, 256, 1, 1.0, t, None, None) == 0
+    assert s.bfly_c2c_scaled(i, o, 65536, 2, 1, 1.0, t, w, None) == 0
+    assert s.bfly_c2c_scaled(i, i, 4096, 1, 1, 1.0, t, None, None) == bfly.ERR_SHAPE  # in place
+
+
+@pytest.mark.parametrize("norm", [None, "backward", "ortho", "forward"])
+def test_norm_scale_follows_numpy(norm):
+    from bfly import ops
+    rng = np.random.default_rng(5)
+    n = 32
+    x = rng.standard_normal(n) + 1j * rng.standard_normal(n)
+    for sign, fn in ((-1, np.fft.fft), (1, np.fft.ifft)):
+        sc = ops.norm_scale(n, sign, norm)
+        raw = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n  # unnormalised both ways
+        want = fn(x, norm=norm) if norm is not None else raw
+        np.testing.assert_allclose(raw * (1.0 if sc is None else sc), want, rtol=1e-12, atol=1e-12)
+    with pytest.raises(ValueError):
+        ops.norm_scale(n, 1, "unitary")
+
+
 def test_chain_launch_count_and_traced_guards_through_stub():
     s = bfly.stub()
     # one launch up to 4096, one workgroup per transform for 8192 / 16384

@@ -213,6 +213,53 @@ def test_real_lds_twiddle_variant_fp64(n):
             assert np.max(np.abs(z.cpu().numpy() - x)) < 1e-5
 
 
+@pytest.mark.parametrize("n,b", [(16, 3), (256, 97), (4096, 40), (8192, 5), (16384, 64), (16384, 256),
+                                 (32768, 3), (65536, 2)])
+def test_c2c_scaled_equals_its_variant_times_scale_bit_for_bit(n, b):
+    # bfly_c2c_scaled multiplies in the last store of the variant it runs
+    # (LDS-twiddle streaming up to 8192, AUTO above: the one-workgroup
+    # kernel, or the four-step chain at 16384 b=64 and 32768 / 65536), so
+    # its output is fl(scale * that variant's output) bit for bit.
+    import torch
+    from bfly import _lib
+    ops, lib = _ops(), _lib.hip()
+    x = torch.from_numpy(oracle.uniform_input(11, (b, 2 * n))).cuda()
+    tab = ops.table(_lib.KIND_C2C, n, x.device)
+    work = ops.workspace(_lib.KIND_C2C, n, b, x.device)
+    wp = work.data_ptr() if work is not None else None
+    st = torch.cuda.current_stream().cuda_stream
+    variant = _lib.VARIANT_STREAM_LT if n <= 8192 else _lib.VARIANT_AUTO
+    for sign, scale in ((1, 1.0 / n), (-1, 1.0 / np.sqrt(n)), (1, -3.0), (-1, 1.0)):
+        ref = torch.empty_like(x)
+        got = torch.full_like(x, float("nan"))
+        assert lib.bfly_c2c_ex(x.data_ptr(), ref.data_ptr(), n, b, sign, variant, tab.data_ptr(), wp, st) == 0
+        assert lib.bfly_c2c_scaled(x.data_ptr(), got.data_ptr(), n, b, sign, scale, tab.data_ptr(), wp, st) == 0
+        torch.cuda.synchronize()
+        want = ref.cpu().numpy() * np.float32(scale)
+        assert np.array_equal(got.cpu().numpy(), want), (n, b, sign, scale)
+        if sign == 1 and scale == 1.0 / n:  # the normalised inverse against FP64
+            gate_c2c(got.cpu().numpy() * n, x.cpu().numpy(), n, sign=+1)
+
+
+@pytest.mark.parametrize("norm", ["backward", "ortho", "forward"])
+@pytest.mark.parametrize("n,b", [(64, 7), (4096, 16), (16384, 4), (65536, 2)])
+def test_c2c_norm_matches_numpy_and_round_trips(n, b, norm):
+    # ops.c2c(norm=...) against numpy.fft's own normalisation of the FP64
+    # transform, and forward then inverse under one convention returns x.
+    import torch
+    x = oracle.uniform_input(12, (b, 2 * n))
+    xc = x.astype(np.float64).view(np.complex128)
+    t = torch.from_numpy(x).cuda().view(torch.complex64)
+    f = _ops().c2c(t, -1, norm=norm)
+    back = _ops().c2c(f, +1, norm=norm)
+    torch.cuda.synchronize()
+    ref = np.fft.fft(xc, axis=-1, norm=norm)
+    got = f.cpu().numpy().astype(np.complex128)
+    e = np.sqrt(np.sum(np.abs(got - ref) ** 2, axis=-1) / np.sum(np.abs(ref) ** 2, axis=-1)).max()
+    assert e <= REL_L2, f"n={n} norm={norm} rel-L2 {e:.3e}"
+    assert np.max(np.abs(back.cpu().numpy() - xc)) < 1e-5
+
+
 @pytest.mark.parametrize("n", [16, 32, 64, 1024, 4096])
 def test_c2c_inverse_is_conjugate_transform(n):
     b = 5
