@@ -62,7 +62,7 @@ SYMBOLS = (
     "bfly_rt_memcpy_h2d_async", "bfly_rt_memcpy_d2h_async", "bfly_rt_malloc_host", "bfly_rt_free_host",
     "bfly_rt_event_create", "bfly_rt_event_destroy", "bfly_rt_event_record", "bfly_rt_event_elapsed",
     "bfly_graph_begin", "bfly_graph_end", "bfly_graph_launch", "bfly_graph_destroy",
-    "bfly_c2c_launches", "bfly_c2c_traced", "bfly_c2c_columns_workspace_bytes", "bfly_c2c_columns",
+    "bfly_c2c_launches", "bfly_c2c_traced", "bfly_c2c_columns_workspace_bytes", "bfly_c2c_columns", "bfly_c2c_rows",
     "bfly_fft2d_table_floats", "bfly_fft2d_table_fill",
     "bfly_fft2d_workspace_bytes", "bfly_fft2d", "bfly_rt_time_transform", "bfly_supported", "bfly_table_floats", "bfly_table_fill",
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_fwd", "bfly_c2c_inv", "bfly_c2c_scaled", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
@@ -132,6 +132,7 @@ def _bind(lib):
         "bfly_c2c_fwd": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_inv": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_scaled": (i32, [vp, vp, u32, u32, i32, C.c_float, vp, vp, vp]),
+        "bfly_c2c_rows": (i32, [vp, C.c_uint64, vp, C.c_uint64, u32, u32, i32, vp, vp]),
         "bfly_rt_teardown": (i32, []),
         "bfly_r2c": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),

@@ -114,6 +114,35 @@ def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None) -> torch.Tenso
     return out
 
 
+def c2c_rows(x: torch.Tensor, sign: int = -1, out: torch.Tensor | None = None) -> torch.Tensor:
+    """Batched C2C over the rows of a 2-D complex64 view whose rows may be
+    padded: stride(1) == 1 and stride(0) >= n, e.g. buf[:, :n] of a
+    [batch][dist] buffer.  The result goes to `out` (a [batch][n] view of the
+    same kind, its gaps untouched) or to a new contiguous tensor
+    (bfly_c2c_rows; unnormalised, sign -1 forward, +1 inverse)."""
+    if not x.is_cuda:
+        raise ValueError("x must be a device tensor")
+    if x.dtype != torch.complex64 or x.dim() != 2:
+        raise TypeError("c2c_rows takes a 2-D complex64 tensor")
+    batch, n = x.shape
+    if out is None:
+        out = torch.empty((batch, n), dtype=torch.complex64, device=x.device)
+    if out.dtype != torch.complex64 or tuple(out.shape) != (batch, n) or out.device != x.device:
+        raise ValueError("out must be a complex64 [batch][n] tensor on the input's device")
+    if batch == 0:
+        return out
+    for t, name in ((x, "x"), (out, "out")):
+        if t.stride(1) != 1 or (batch > 1 and t.stride(0) < n):
+            raise ValueError(f"{name} rows must be unit-stride and at least n apart")
+    idist = x.stride(0) if batch > 1 else n
+    odist = out.stride(0) if batch > 1 else n
+    tab = table(_lib.KIND_C2C, n, x.device)
+    rc = _lib.hip().bfly_c2c_rows(x.data_ptr(), idist, out.data_ptr(), odist, n, batch, sign, tab.data_ptr(),
+                                  _stream(x.device))
+    _lib.check(rc, f"bfly_c2c_rows(n={n}, batch={batch}, idist={idist}, odist={odist}, sign={sign})")
+    return out
+
+
 def table2d_cols(n1: int, device) -> torch.Tensor:
     """Device copy of the 2-D column table of n1 (bfly_fft2d_table_fill), cached."""
     device = torch.device(device)

@@ -186,6 +186,12 @@ int bfly_c2c_columns(const float* in, float* out, uint32_t n, uint32_t ncols, ui
     return launched(bfly_kern::columns(in, out, n, ncols, batch, sign, table_cols, work, S(stream)));
 }
 
+int bfly_c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
+                  int sign, const float* table, void* stream) {
+    if (int g = bfly_host::check_rows(sign, in, idist, out, odist, n, batch, table)) return g;
+    return launched(bfly_kern::c2c_rows(in, idist, out, odist, n, batch, sign, table, S(stream)));
+}
+
 size_t bfly_fft2d_table_floats(uint32_t n1) { return 2 * bfly_host::fft2d_layout(n1).complex_len; }
 
 int bfly_fft2d_table_fill(uint32_t n1, float* host_table) { return bfly_host::fft2d_fill(n1, host_table); }

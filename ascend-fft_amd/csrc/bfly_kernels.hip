@@ -169,7 +169,7 @@ constexpr int lt_wg() { return PlanOf<I>::T > bfly_plan::kBlock ? PlanOf<I>::T :
 template <int I, int DIR>
 __global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
     const float2* __restrict__ in, float2* __restrict__ out, const float2* __restrict__ tw,
-    const float2* __restrict__ tw2g, uint32_t batch, float scale) {
+    const float2* __restrict__ tw2g, uint32_t batch, float scale, uint64_t idist, uint64_t odist) {
     using PL = PlanOf<I>;
     constexpr int WG = lt_wg<I>();
     constexpr int N = PL::N, T = PL::T, B = WG / T;
@@ -197,7 +197,7 @@ __global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
     uint32_t g = blockIdx.x;
     if (g < groups) {
         const uint64_t row = (uint64_t)g * B + bl;
-        const float2* src = in + row * N;
+        const float2* src = in + row * idist;
 #pragma unroll
         for (int m = 0; m < P / R0; m++)
 #pragma unroll
@@ -211,7 +211,7 @@ __global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
         const uint32_t gn = g + gridDim.x;
         if (gn < groups) {
             const uint64_t row = (uint64_t)gn * B + bl;
-            const float2* src = in + row * N;
+            const float2* src = in + row * idist;
 #pragma unroll
             for (int m = 0; m < P / R0; m++)
 #pragma unroll
@@ -226,7 +226,7 @@ __global__ __launch_bounds__(lt_wg<I>()) void k_c2c_stream_lt(
             for (int i = 0; i < P; i++) cur[i] = make_float2(cur[i].x * scale, cur[i].y * scale);
         }
         if (row < batch) {
-            float2* dst = out + row * N;
+            float2* dst = out + row * odist;
 #pragma unroll
             for (int m = 0; m < P / RL; m++)
 #pragma unroll
@@ -536,14 +536,19 @@ hipError_t launch_c2c_dir(const float2* in, float2* out, const float2* tw, uint3
 
 template <int I, int DIR>
 hipError_t launch_c2c_lt(const float2* in, float2* out, const float2* tw, const float2* tw2g,
-                         uint32_t batch, hipStream_t s, float scale = 1.0f) {
+                         uint32_t batch, hipStream_t s, float scale = 1.0f, uint64_t idist = 0,
+                         uint64_t odist = 0) {
     constexpr int WG = lt_wg<I>(), B = WG / PlanOf<I>::T;
+    // row distances in complex values; 0 = contiguous rows of n
+    if (!idist) idist = PlanOf<I>::N;
+    if (!odist) odist = PlanOf<I>::N;
     const uint64_t groups = (batch + B - 1) / B;
     static bfly_kern::ResidentCache rc;
     const int cap = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_c2c_stream_lt<I, DIR>), WG);
     if (cap <= 0) return hipErrorInvalidValue;
     const dim3 grid((unsigned)bfly_plan::stream_grid(groups, (uint64_t)cap));
-    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(WG), 0, s, in, out, tw, tw2g, batch, scale);
+    hipLaunchKernelGGL((k_c2c_stream_lt<I, DIR>), grid, dim3(WG), 0, s, in, out, tw, tw2g, batch, scale, idist,
+                       odist);
     return hipGetLastError();
 }
 
@@ -746,6 +751,40 @@ hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign
         case 16384: return launch_long<idx(128), idx(128)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
         case 32768: return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
         case 65536: return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, mid, scale);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+template <int I>
+hipError_t launch_rows(const float2* in, uint64_t idist, float2* out, uint64_t odist, const float2* tw,
+                       uint32_t batch, int sign, hipStream_t s) {
+    constexpr int N = PlanOf<I>::N;
+    const bfly_host::TableLayout L = bfly_host::table_layout(0, N);
+    // up to 4096 the local table with [A | B] at `extra`; 8192 / 16384 the
+    // one-workgroup layout (`single`) with the full W_N table at `extra`
+    const float2* ts = N > bfly_plan::kMaxLocalN ? tw + L.single : tw;
+    const float2* wN = tw + L.extra;
+    return sign < 0 ? launch_c2c_lt<I, -1>(in, out, ts, wN, batch, s, 1.0f, idist, odist)
+                    : launch_c2c_lt<I, 1>(in, out, ts, wN, batch, s, 1.0f, idist, odist);
+}
+
+hipError_t c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
+                    int sign, const float* table, hipStream_t s) {
+    const float2* i2 = reinterpret_cast<const float2*>(in);
+    float2* o2 = reinterpret_cast<float2*>(out);
+    const float2* tw = reinterpret_cast<const float2*>(table);
+    switch (n) {
+        case 16: return launch_rows<idx(16)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 32: return launch_rows<idx(32)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 64: return launch_rows<idx(64)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 128: return launch_rows<idx(128)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 256: return launch_rows<idx(256)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 512: return launch_rows<idx(512)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 1024: return launch_rows<idx(1024)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 2048: return launch_rows<idx(2048)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 4096: return launch_rows<idx(4096)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 8192: return launch_rows<idx(8192)>(i2, idist, o2, odist, tw, batch, sign, s);
+        case 16384: return launch_rows<idx(16384)>(i2, idist, o2, odist, tw, batch, sign, s);
         default: return hipErrorInvalidValue;
     }
 }

@@ -151,6 +151,11 @@ int bfly_c2c_columns(const float* in, float* out, uint32_t n, uint32_t ncols, ui
     if (int g = bfly_host::check_columns(sign, in, out, n, ncols, batch, table_cols, work)) return g;
     return launch();
 }
+int bfly_c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
+                  int sign, const float* table, void*) {
+    if (int g = bfly_host::check_rows(sign, in, idist, out, odist, n, batch, table)) return g;
+    return launch();
+}
 size_t bfly_fft2d_table_floats(uint32_t n1) { return 2 * bfly_host::fft2d_layout(n1).complex_len; }
 int bfly_fft2d_table_fill(uint32_t n1, float* host_table) { return bfly_host::fft2d_fill(n1, host_table); }
 size_t bfly_fft2d_workspace_bytes(uint32_t n1, uint32_t n2, uint32_t batch) {

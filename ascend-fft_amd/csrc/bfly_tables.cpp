@@ -140,6 +140,24 @@ int check_columns(int sign, const void* in, const void* out, uint32_t n1, uint32
     return BFLY_OK;
 }
 
+int check_rows(int sign, const void* in, uint64_t idist, const void* out, uint64_t odist, uint32_t n,
+               uint32_t batch, const void* table) {
+    if (!in || !out || batch == 0 || !shape_ok(BFLY_KIND_C2C, n) || n > (uint32_t)bfly_plan::kMaxSingleN)
+        return BFLY_ERR_SHAPE;
+    if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
+    if (idist < n || odist < n) return BFLY_ERR_SHAPE;
+    const auto lo = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
+    if ((lo(in) | lo(out)) % 8) return BFLY_ERR_SHAPE;  // float2 rows
+    // extents in complex values, kept below 2^58 so byte offsets fit
+    constexpr uint64_t kMaxExtent = 1ull << 58;
+    const uint64_t b1 = batch - 1ull;
+    if (b1 && (idist > (kMaxExtent - n) / b1 || odist > (kMaxExtent - n) / b1)) return BFLY_ERR_SHAPE;
+    if (!table) return BFLY_ERR_TABLE;
+    const uint64_t in_bytes = 8 * (b1 * idist + n), out_bytes = 8 * (b1 * odist + n);
+    if (!(lo(in) + in_bytes <= lo(out) || lo(out) + out_bytes <= lo(in))) return BFLY_ERR_SHAPE;
+    return BFLY_OK;
+}
+
 int check_fft2d(int sign, const void* in, const void* out, uint32_t n1, uint32_t n2, uint32_t batch,
                 const void* table_rows, const void* table_cols, const void* work) {
     if (!fft2d_ok(n1, n2) || batch == 0 || !in || !out || !work) return BFLY_ERR_SHAPE;

@@ -14,7 +14,8 @@ the whole batch ([4096][8192] float32, interleaved), inputs resident in HBM.  Pr
                  over the plain one-shot variant and its relative error
   rows           the other hot-path rows at their shapes: C2C n=1024 b=64
                  latency, R2C / C2R 4096x4096 (reference CPU rows beside
-                 them), long C2C 65536 (b=64, 256) and 16384 (b=1024)
+                 them), long C2C 65536 (b=64, 256) and 16384 (b=1024), padded-row
+                 C2C and the normalised inverse at 4096 x 4096
   patterns       the uneven batched workload (512 requests) as one grouped
                  persistent launch under hipGraph vs per-request launches:
                  transforms/s, GB/s, p50/p99 per-request latency; the
@@ -240,6 +241,20 @@ def secondary_rows(cpu_seconds: float, reps: int = 20, rounds: int = 3):
         out[f"c2c_n{n}_b{b}"] = row(timed(lambda i: _lib.check(lib.bfly_c2c(*args[i % 4]), "long")),
                                     16 * n * b, b)
         del xs, ys, work
+    # padded rows (transforms 4096 + 64 complex values apart, gaps untouched)
+    # and the normalised inverse (scale 1/n in the last store), both at the
+    # headline shape
+    n, b, d = 4096, 4096, 4096 + 64
+    xs = [torch.rand(b, 2 * d, device=dev) for _ in range(4)]
+    ys = [torch.empty_like(xs[0]) for _ in range(4)]
+    tab = ops.table(_lib.KIND_C2C, n, dev)
+    args = [(xs[i].data_ptr(), d, ys[i].data_ptr(), d, n, b, -1, tab.data_ptr(), sid()) for i in range(4)]
+    out["c2c_rows_n4096_b4096_dist4160"] = row(
+        timed(lambda i: _lib.check(lib.bfly_c2c_rows(*args[i % 4]), "rows")), 16 * n * b, b)
+    args = [(xs[i].data_ptr(), ys[i].data_ptr(), n, b, 1, 1.0 / n, tab.data_ptr(), None, sid()) for i in range(4)]
+    out["c2c_inv_normalised_n4096_b4096"] = row(
+        timed(lambda i: _lib.check(lib.bfly_c2c_scaled(*args[i % 4]), "scaled")), 16 * n * b, b)
+    del xs, ys
     return out
 
 

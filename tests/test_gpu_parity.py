@@ -241,6 +241,41 @@ def test_c2c_scaled_equals_its_variant_times_scale_bit_for_bit(n, b):
             gate_c2c(got.cpu().numpy() * n, x.cpu().numpy(), n, sign=+1)
 
 
+@pytest.mark.parametrize("n,b,ipad,opad", [(16, 5, 1, 3), (256, 97, 8, 0), (4096, 40, 3, 64), (8192, 6, 1, 1),
+                                           (16384, 3, 2, 0), (1024, 1, 5, 5)])
+def test_c2c_rows_padded_equals_contiguous_bit_for_bit(n, b, ipad, opad):
+    # bfly_c2c_rows over rows n + ipad apart into rows n + opad apart: the
+    # LDS-twiddle streaming kernel with row distances, so it equals the
+    # contiguous STREAM_LT call bit for bit.  NaN in the input gaps would
+    # reach the result if a gap were read; the output gaps keep their fill.
+    import torch
+    from bfly import _lib
+    ops, lib = _ops(), _lib.hip()
+    x = oracle.uniform_input(13, (b, 2 * n))
+    xt = torch.from_numpy(x).cuda().view(torch.complex64)
+    nan = complex(float("nan"), float("nan"))
+    ibuf = torch.full((b, n + ipad), nan, dtype=torch.complex64, device="cuda")
+    ibuf[:, :n] = xt
+    tab = ops.table(_lib.KIND_C2C, n, xt.device)
+    st = torch.cuda.current_stream().cuda_stream
+    for sign in (-1, 1):
+        obuf = torch.full((b, n + opad), complex(7.0, -7.0), dtype=torch.complex64, device="cuda")
+        ops.c2c_rows(ibuf[:, :n], sign, out=obuf[:, :n])
+        ref = torch.empty_like(xt)
+        assert lib.bfly_c2c_ex(xt.data_ptr(), ref.data_ptr(), n, b, sign, _lib.VARIANT_STREAM_LT, tab.data_ptr(),
+                               None, st) == 0
+        torch.cuda.synchronize()
+        got = torch.view_as_real(obuf[:, :n]).cpu()
+        assert torch.equal(got, torch.view_as_real(ref).cpu()), (n, b, sign)
+        if opad:
+            assert bool((obuf[:, n:] == complex(7.0, -7.0)).all())
+        gate_c2c(got.numpy().reshape(b, 2 * n), x, n, sign=sign)
+    # a new contiguous result when no `out` is given
+    y = ops.c2c_rows(ibuf[:, :n])
+    torch.cuda.synchronize()
+    gate_c2c(torch.view_as_real(y).cpu().numpy().reshape(b, 2 * n), x, n)
+
+
 @pytest.mark.parametrize("norm", ["backward", "ortho", "forward"])
 @pytest.mark.parametrize("n,b", [(64, 7), (4096, 16), (16384, 4), (65536, 2)])
 def test_c2c_norm_matches_numpy_and_round_trips(n, b, norm):
