@@ -101,9 +101,10 @@ BFLY_API int bfly_rt_time_transform(int kind, int sign, const float* in, float* 
  * graph: bfly_graph_begin starts capturing the stream (thread-local mode;
  * nothing executes while capturing), bfly_graph_end ends the capture and
  * instantiates it into *graph, bfly_graph_launch replays it on a stream and
- * bfly_graph_destroy frees it.  Capturable: bfly_c2c / bfly_c2c_ex,
- * bfly_r2c / bfly_c2r / their _ex forms, bfly_real_ex, bfly_fft2d,
- * bfly_c2c_columns, bfly_mode_mix, bfly_group_launch and the _async copies;
+ * bfly_graph_destroy frees it.  Capturable: bfly_c2c / bfly_c2c_ex /
+ * bfly_c2c_scaled / bfly_c2c_rows, bfly_r2c / bfly_c2r / their _ex forms,
+ * bfly_real_ex, bfly_fft2d, bfly_c2c_columns, bfly_mode_mix,
+ * bfly_group_launch and the _async copies;
  * not the synchronising helpers (bfly_rt_memcpy_h2d / _d2h,
  * bfly_rt_stream_sync, bfly_rt_time_transform, bfly_rt_event_elapsed).
  * `stream` must be a created stream, not the null stream.  No counterpart in

@@ -1,7 +1,8 @@
 """GPU: hipGraph capture of library calls through the C ABI (bfly_graph_*).
 
 One graph holds a C2C (streaming LDS-twiddle kernel), a four-step long C2C,
-an R2C, a C2R, a 2-D C2C and a strided column transform.  It is replayed
+an R2C, a C2R, a 2-D C2C, a strided column transform, a normalised inverse
+and a padded-row C2C.  It is replayed
 several times with the program's own eager work between the replays (new
 inputs written, the same calls run eagerly into separate outputs, full
 synchronisations), and every replay must equal the eager results bit for
@@ -34,9 +35,10 @@ def test_graph_replays_equal_eager_calls(dev):
     hr = rnd(32, 4098)
     x2 = rnd(3, 512, 2 * 256)
     xc = rnd(2, 1024, 2 * 33)
-    ins = [x1, xl, xr, hr, x2, xc]
+    xp = rnd(16, 2 * (1024 + 8))  # padded rows: 1024 values every 1032
+    ins = [x1, xl, xr, hr, x2, xc, xp]
     outs = [torch.empty_like(x1), torch.empty_like(xl), torch.empty_like(hr), torch.empty_like(xr),
-            torch.empty_like(x2), torch.empty_like(xc)]
+            torch.empty_like(x2), torch.empty_like(xc), torch.empty_like(x1), torch.empty(16, 2 * 1024, device=dev)]
     eager = [torch.empty_like(o) for o in outs]
     t1 = ops.table(_lib.KIND_C2C, 4096, dev)
     tl = ops.table(_lib.KIND_C2C, 65536, dev)
@@ -46,6 +48,7 @@ def test_graph_replays_equal_eager_calls(dev):
     w2 = torch.empty(2 * x2.numel(), device=dev)
     tcc = ops.table2d_cols(1024, dev)
     wc = torch.empty(2 * xc.numel(), device=dev)
+    tp = ops.table(_lib.KIND_C2C, 1024, dev)
 
     def calls(o):
         rc = [lib.bfly_c2c(x1.data_ptr(), o[0].data_ptr(), 4096, 64, -1, t1.data_ptr(), None, sid),
@@ -55,8 +58,10 @@ def test_graph_replays_equal_eager_calls(dev):
               lib.bfly_fft2d(x2.data_ptr(), o[4].data_ptr(), 512, 256, 3, -1, t2r.data_ptr(), t2c.data_ptr(),
                              w2.data_ptr(), sid),
               lib.bfly_c2c_columns(xc.data_ptr(), o[5].data_ptr(), 1024, 33, 2, 1, tcc.data_ptr(), wc.data_ptr(),
-                                   sid)]
-        assert rc == [0] * 6, rc
+                                   sid),
+              lib.bfly_c2c_scaled(x1.data_ptr(), o[6].data_ptr(), 4096, 64, 1, 1.0 / 4096, t1.data_ptr(), None, sid),
+              lib.bfly_c2c_rows(xp.data_ptr(), 1032, o[7].data_ptr(), 1024, 1024, 16, -1, tp.data_ptr(), sid)]
+        assert rc == [0] * 8, rc
 
     torch.cuda.synchronize()
     with torch.cuda.stream(s):
