@@ -2,6 +2,7 @@
 
 Adjoints (unnormalised forward F, conjugate-Wirtinger gradients as torch uses):
   c2c(x, s)     y = F_s x              grad_x = F_{-s} grad_y          (F^H = F_{-s})
+  c2c(x, s, a)  y = a F_s x (a real)   grad_x = a F_{-s} grad_y        (numpy's `norm` scales)
   rfft(x)       y = (F x)[0..n/2]      grad_x = n * irfft(w . grad_y), w = 1 at bins 0 and n/2,
                                        1/2 elsewhere (the Hermitian weights of the real input)
   irfft(h, n)   x = irfft(h)           grad_h = rfft(grad_x) * v / n, v = 1 at bins 0 and n/2,
@@ -26,14 +27,17 @@ def _as_complex(f: torch.Tensor) -> torch.Tensor:
 
 
 class C2C(torch.autograd.Function):
+    """y = scale * F_sign x along the last axis (scale None: unnormalised);
+    the backward is the same scale times the conjugate transform, both with
+    the scale applied in the kernel's last store."""
     @staticmethod
-    def forward(ctx, x: torch.Tensor, sign: int = -1):
-        ctx.sign = sign
-        return ops.c2c(x.contiguous(), sign)
+    def forward(ctx, x: torch.Tensor, sign: int = -1, scale: float | None = None):
+        ctx.sign, ctx.scale = sign, scale
+        return ops.c2c(x.contiguous(), sign, scale=scale)
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
-        return ops.c2c(g.contiguous(), -ctx.sign), None
+        return ops.c2c(g.contiguous(), -ctx.sign, scale=ctx.scale), None, None
 
 
 class RFFT(torch.autograd.Function):
@@ -96,9 +100,17 @@ def irfft_modes(h: torch.Tensor, n: int) -> torch.Tensor:
     return IRFFTModes.apply(h, n)
 
 
-def fft(x: torch.Tensor) -> torch.Tensor:
-    """Differentiable forward C2C of complex64 [..., n]."""
-    return C2C.apply(x, -1)
+def fft(x: torch.Tensor, norm: str | None = None) -> torch.Tensor:
+    """Differentiable forward C2C of complex64 [..., n]; `norm` as in
+    torch.fft.fft (None and "backward": unscaled, "ortho": 1/sqrt(n),
+    "forward": 1/n)."""
+    return C2C.apply(x, -1, ops.norm_scale(x.shape[-1], -1, norm))
+
+
+def ifft(x: torch.Tensor, norm: str | None = "backward") -> torch.Tensor:
+    """Differentiable inverse C2C; `norm` as in torch.fft.ifft ("backward":
+    1/n, "ortho": 1/sqrt(n), "forward": unscaled; None: unscaled)."""
+    return C2C.apply(x, 1, ops.norm_scale(x.shape[-1], 1, norm))
 
 
 def ifft_unnormalized(x: torch.Tensor) -> torch.Tensor:

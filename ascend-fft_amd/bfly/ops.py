@@ -90,10 +90,12 @@ def norm_scale(n: int, sign: int, norm: str | None) -> float | None:
     return 1.0 / n if (norm == "backward") == inverse else None
 
 
-def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None) -> torch.Tensor:
+def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None, *, scale: float | None = None) -> torch.Tensor:
     """C2C along the last axis (sign -1 forward, +1 inverse); unnormalised
-    unless `norm` names numpy's convention, whose scale the transform's last
-    store applies."""
+    unless `norm` names numpy's convention or `scale` gives the factor
+    outright, which the transform's last store applies."""
+    if norm is not None and scale is not None:
+        raise ValueError("give norm or scale, not both")
     _check_device(x, "x")
     if x.dtype == torch.complex64:
         n = x.shape[-1]
@@ -110,7 +112,7 @@ def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None) -> torch.Tenso
     if batch == 0:
         return out
     of = torch.view_as_real(out) if x.dtype == torch.complex64 else out
-    c2c_raw(xf, of, n, batch, sign, scale=norm_scale(n, sign, norm))
+    c2c_raw(xf, of, n, batch, sign, scale=scale if scale is not None else norm_scale(n, sign, norm))
     return out
 
 

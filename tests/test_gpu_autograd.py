@@ -35,6 +35,27 @@ def test_c2c_adjoint_matches_fp64_autograd(n):
     assert _rel(xd.grad.cpu().numpy(), x64.grad.numpy()) < 1e-6
 
 
+@pytest.mark.parametrize("norm", ["backward", "ortho", "forward"])
+@pytest.mark.parametrize("n", [64, 4096, 16384])
+def test_normalised_c2c_gradients_match_torch_fft_fp64(n, norm):
+    # fft / ifft with torch.fft's `norm` conventions: values and gradients
+    # against torch.fft.fft / ifft autograd in FP64 on the CPU.
+    import torch
+    from bfly import autograd as bag
+    g = torch.Generator().manual_seed(1)
+    x = torch.randn(3, n, dtype=torch.complex64, generator=g)
+    up = torch.randn(3, n, dtype=torch.complex64, generator=g)
+    for ours, ref in ((bag.fft, torch.fft.fft), (bag.ifft, torch.fft.ifft)):
+        xd = x.cuda().requires_grad_(True)
+        y = ours(xd, norm=norm)
+        (y * up.cuda().conj()).real.sum().backward()
+        x64 = x.to(torch.complex128).requires_grad_(True)
+        y64 = ref(x64, norm=norm)
+        (y64 * up.to(torch.complex128).conj()).real.sum().backward()
+        assert _rel(y.detach().cpu().numpy(), y64.detach().numpy()) < 1e-6, (ours.__name__, norm)
+        assert _rel(xd.grad.cpu().numpy(), x64.grad.numpy()) < 1e-6, (ours.__name__, norm)
+
+
 @pytest.mark.parametrize("n", [128, 1024, 4096])
 def test_rfft_irfft_adjoints_match_fp64_autograd(n):
     import torch
