@@ -132,7 +132,7 @@ def _bind(lib):
         "bfly_c2c_fwd": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_inv": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_scaled": (i32, [vp, vp, u32, u32, i32, C.c_float, vp, vp, vp]),
-        "bfly_c2c_rows": (i32, [vp, C.c_uint64, vp, C.c_uint64, u32, u32, i32, vp, vp]),
+        "bfly_c2c_rows": (i32, [vp, C.c_uint64, vp, C.c_uint64, u32, u32, i32, vp, vp, vp]),
         "bfly_rt_teardown": (i32, []),
         "bfly_r2c": (i32, [vp, vp, u32, u32, vp, vp]),
         "bfly_c2r": (i32, [vp, vp, u32, u32, vp, vp]),

@@ -139,8 +139,9 @@ def c2c_rows(x: torch.Tensor, sign: int = -1, out: torch.Tensor | None = None) -
     idist = x.stride(0) if batch > 1 else n
     odist = out.stride(0) if batch > 1 else n
     tab = table(_lib.KIND_C2C, n, x.device)
+    work = workspace(_lib.KIND_C2C, n, batch, x.device) if n > 16384 else None
     rc = _lib.hip().bfly_c2c_rows(x.data_ptr(), idist, out.data_ptr(), odist, n, batch, sign, tab.data_ptr(),
-                                  _stream(x.device))
+                                  work.data_ptr() if work is not None else None, _stream(x.device))
     _lib.check(rc, f"bfly_c2c_rows(n={n}, batch={batch}, idist={idist}, odist={odist}, sign={sign})")
     return out
 

@@ -187,9 +187,9 @@ int bfly_c2c_columns(const float* in, float* out, uint32_t n, uint32_t ncols, ui
 }
 
 int bfly_c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
-                  int sign, const float* table, void* stream) {
-    if (int g = bfly_host::check_rows(sign, in, idist, out, odist, n, batch, table)) return g;
-    return launched(bfly_kern::c2c_rows(in, idist, out, odist, n, batch, sign, table, S(stream)));
+                  int sign, const float* table, void* work, void* stream) {
+    if (int g = bfly_host::check_rows(sign, in, idist, out, odist, n, batch, table, work)) return g;
+    return launched(bfly_kern::c2c_rows(in, idist, out, odist, n, batch, sign, table, work, S(stream)));
 }
 
 size_t bfly_fft2d_table_floats(uint32_t n1) { return 2 * bfly_host::fft2d_layout(n1).complex_len; }

@@ -409,7 +409,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float
 template <int I1, int N2, int DIR, int WG = bfly_plan::kBlock>
 __global__ __launch_bounds__(WG) void k_long_cols(
     const float2* __restrict__ in, float2* __restrict__ ws, const float2* __restrict__ tw1,
-    const float2* __restrict__ wN) {
+    const float2* __restrict__ wN, uint64_t idist) {
     using PL = PlanOf<I1>;
     constexpr int N1 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -433,7 +433,7 @@ __global__ __launch_bounds__(WG) void k_long_cols(
     const uint64_t b = blockIdx.x / GROUPS;
     const int n2 = (blockIdx.x % GROUPS) * C + c;
     constexpr uint64_t N = (uint64_t)N1 * N2;
-    const float2* src = in + b * N + n2;
+    const float2* src = in + b * idist + n2;
     float2 v[P];
 #pragma unroll
     for (int m = 0; m < P / R0; m++)
@@ -457,7 +457,8 @@ __global__ __launch_bounds__(WG) void k_long_cols(
 
 template <int I2, int N1, int DIR, int WG = bfly_plan::kBlock>
 __global__ __launch_bounds__(WG) void k_long_rows(
-    const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2, float scale) {
+    const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2, float scale,
+    uint64_t odist) {
     using PL = PlanOf<I2>;
     constexpr int N2 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -492,7 +493,7 @@ __global__ __launch_bounds__(WG) void k_long_rows(
         for (int r = 0; r < R0; r++) v[m * R0 + r] = L[lds_idx(t + m * T + r * (N2 / R0))];
     __syncthreads();
     run_plan_lt<I2, T, DIR, LX_PAD>(v, L, 0, t1s, t1s, t1s, t, std::make_index_sequence<2>{});
-    float2* dst = out + b * N + k1_0 + c;
+    float2* dst = out + b * odist + k1_0 + c;
     if (scale != 1.0f) {  // uniform: the normalised entry point only
 #pragma unroll
         for (int i = 0; i < P; i++) v[i] = make_float2(v[i].x * scale, v[i].y * scale);
@@ -648,8 +649,11 @@ hipError_t launch_c2c_single(const float2* in, float2* out, const float2* tw, ui
 template <int I1, int I2, int DIR>
 hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const float2* tw1,
                            const float2* tw2, const float2* wN, uint32_t batch, hipStream_t s,
-                           hipEvent_t mid, float scale) {
+                           hipEvent_t mid, float scale, uint64_t idist, uint64_t odist) {
     constexpr int N1 = kPlans[I1].n, N2 = kPlans[I2].n;
+    // row distances in complex values; 0 = contiguous rows of n
+    if (!idist) idist = (uint64_t)N1 * N2;
+    if (!odist) odist = (uint64_t)N1 * N2;
     // Column kernel in 512-thread workgroups when that still fills every
     // resident slot: twice the columns per workgroup, 256-byte runs (65536 x
     // 256: 98.8-100.9 vs 103.1-104.5 us; 65536 x 64: 25.7 vs 31.7; 32768 x
@@ -662,24 +666,24 @@ hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const floa
     const int fit = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_long_cols<I1, N2, DIR, 512>), 512);
     if (fit > 0 && (uint64_t)batch * (N2 / C1w) >= (uint64_t)fit)
         hipLaunchKernelGGL((k_long_cols<I1, N2, DIR, 512>), dim3(batch * (N2 / C1w)), dim3(512), 0, s, in, ws,
-                           tw1, wN);
+                           tw1, wN, idist);
     else
         hipLaunchKernelGGL((k_long_cols<I1, N2, DIR>), dim3(batch * (N2 / C1)), dim3(bfly_plan::kBlock), 0, s,
-                           in, ws, tw1, wN);
+                           in, ws, tw1, wN, idist);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),
-                       0, s, ws, out, tw2, scale);
+                       0, s, ws, out, tw2, scale, odist);
     return hipGetLastError();
 }
 
 template <int I1, int I2>
 hipError_t launch_long(const float2* in, float2* out, float2* ws, const float2* tw1,
                        const float2* tw2, const float2* wN, uint32_t batch, int sign,
-                       hipStream_t s, hipEvent_t mid, float scale) {
-    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s, mid, scale)
-                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s, mid, scale);
+                       hipStream_t s, hipEvent_t mid, float scale, uint64_t idist = 0, uint64_t odist = 0) {
+    return sign < 0 ? launch_long_dir<I1, I2, -1>(in, out, ws, tw1, tw2, wN, batch, s, mid, scale, idist, odist)
+                    : launch_long_dir<I1, I2, 1>(in, out, ws, tw1, tw2, wN, batch, s, mid, scale, idist, odist);
 }
 
 constexpr int idx(int n) { return bfly_plan::plan_index((uint32_t)n); }
@@ -769,7 +773,7 @@ hipError_t launch_rows(const float2* in, uint64_t idist, float2* out, uint64_t o
 }
 
 hipError_t c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
-                    int sign, const float* table, hipStream_t s) {
+                    int sign, const float* table, void* work, hipStream_t s) {
     const float2* i2 = reinterpret_cast<const float2*>(in);
     float2* o2 = reinterpret_cast<float2*>(out);
     const float2* tw = reinterpret_cast<const float2*>(table);
@@ -785,6 +789,18 @@ hipError_t c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist,
         case 4096: return launch_rows<idx(4096)>(i2, idist, o2, odist, tw, batch, sign, s);
         case 8192: return launch_rows<idx(8192)>(i2, idist, o2, odist, tw, batch, sign, s);
         case 16384: return launch_rows<idx(16384)>(i2, idist, o2, odist, tw, batch, sign, s);
+        default: break;
+    }
+    // four-step: the column pass reads the strided input rows, the row pass
+    // writes the strided output rows; the workspace stays contiguous
+    const bfly_host::TableLayout L = bfly_host::table_layout(0, n);
+    const float2 *t1 = tw + L.stage0, *t2 = tw + L.stage1, *wN = tw + L.extra;
+    float2* ws = reinterpret_cast<float2*>(work);
+    switch (n) {
+        case 32768:
+            return launch_long<idx(128), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, nullptr, 1.0f, idist, odist);
+        case 65536:
+            return launch_long<idx(256), idx(256)>(i2, o2, ws, t1, t2, wN, batch, sign, s, nullptr, 1.0f, idist, odist);
         default: return hipErrorInvalidValue;
     }
 }

@@ -23,10 +23,11 @@ int resident(ResidentCache& cache, const void* kernel, int block);
 // every output in the final store; the LDS-twiddle streaming kernels
 // (BFLY_VARIANT_STREAM_LT, and 8192 / 16384 under AUTO) and the four-step
 // chain apply it, the other variants require 1.
-// Batched C2C over rows `idist` / `odist` complex values apart (>= n),
-// through the LDS-twiddle streaming kernel; n 16..16384, `table` as for c2c.
+// Batched C2C over rows `idist` / `odist` complex values apart (>= n):
+// the LDS-twiddle streaming kernel for n 16..16384, the four-step chain
+// (contiguous workspace `work`) for 32768 / 65536; `table` as for c2c.
 hipError_t c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
-                    int sign, const float* table, hipStream_t s);
+                    int sign, const float* table, void* work, hipStream_t s);
 hipError_t c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign, int variant,
                const float* table, void* work, hipStream_t s, hipEvent_t mid = nullptr, float scale = 1.0f);
 // variant: BFLY_VARIANT_PLAIN (one-shot grid), BFLY_VARIANT_STREAM or

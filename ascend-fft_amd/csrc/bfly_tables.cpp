@@ -140,10 +140,14 @@ int check_columns(int sign, const void* in, const void* out, uint32_t n1, uint32
     return BFLY_OK;
 }
 
+static bool grid_ok(int kind, uint32_t n, uint32_t batch);
+
 int check_rows(int sign, const void* in, uint64_t idist, const void* out, uint64_t odist, uint32_t n,
-               uint32_t batch, const void* table) {
-    if (!in || !out || batch == 0 || !shape_ok(BFLY_KIND_C2C, n) || n > (uint32_t)bfly_plan::kMaxSingleN)
+               uint32_t batch, const void* table, const void* work) {
+    if (!in || !out || batch == 0 || !shape_ok(BFLY_KIND_C2C, n) || !grid_ok(BFLY_KIND_C2C, n, batch))
         return BFLY_ERR_SHAPE;
+    const bool four_step = n > (uint32_t)bfly_plan::kMaxSingleN;
+    if (four_step && !work) return BFLY_ERR_SHAPE;
     if (sign != -1 && sign != 1) return BFLY_ERR_SHAPE;
     if (idist < n || odist < n) return BFLY_ERR_SHAPE;
     const auto lo = [](const void* p) { return reinterpret_cast<uintptr_t>(p); };
@@ -154,7 +158,13 @@ int check_rows(int sign, const void* in, uint64_t idist, const void* out, uint64
     if (b1 && (idist > (kMaxExtent - n) / b1 || odist > (kMaxExtent - n) / b1)) return BFLY_ERR_SHAPE;
     if (!table) return BFLY_ERR_TABLE;
     const uint64_t in_bytes = 8 * (b1 * idist + n), out_bytes = 8 * (b1 * odist + n);
-    if (!(lo(in) + in_bytes <= lo(out) || lo(out) + out_bytes <= lo(in))) return BFLY_ERR_SHAPE;
+    const auto apart = [&](const void* p, uint64_t pb, const void* q, uint64_t qb) {
+        return lo(p) + pb <= lo(q) || lo(q) + qb <= lo(p);
+    };
+    if (!apart(in, in_bytes, out, out_bytes)) return BFLY_ERR_SHAPE;
+    const uint64_t work_bytes = 8ull * n * batch;  // contiguous [batch][n]
+    if (four_step && (!apart(work, work_bytes, in, in_bytes) || !apart(work, work_bytes, out, out_bytes)))
+        return BFLY_ERR_SHAPE;
     return BFLY_OK;
 }
 

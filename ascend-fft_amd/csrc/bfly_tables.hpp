@@ -64,11 +64,12 @@ int fft2d_fill(uint32_t n1, float* dst);
 // batch >= 1, disjoint buffers, a workspace when n1 > 256, grids that fit).
 int check_columns(int sign, const void* in, const void* out, uint32_t n1, uint32_t ncols, uint32_t batch,
                   const void* table_cols, const void* work);
-// Guards of bfly_c2c_rows: n a power of two in [16, 16384], batch >= 1,
+// Guards of bfly_c2c_rows: n a power of two in [16, 65536], batch >= 1,
 // row distances >= n (complex values), 8-byte aligned buffers whose spans
-// [p, p + 8 ((batch-1) dist + n)) do not overlap, extents below 2^58.
+// [p, p + 8 ((batch-1) dist + n)) do not overlap, extents below 2^58; above
+// 16384 a workspace of 8 n batch bytes overlapping neither span.
 int check_rows(int sign, const void* in, uint64_t idist, const void* out, uint64_t odist, uint32_t n,
-               uint32_t batch, const void* table);
+               uint32_t batch, const void* table, const void* work);
 // Guards of bfly_fft2d: BFLY_OK or the status code.
 int check_fft2d(int sign, const void* in, const void* out, uint32_t n1, uint32_t n2, uint32_t batch,
                 const void* table_rows, const void* table_cols, const void* work);

@@ -248,7 +248,7 @@ def secondary_rows(cpu_seconds: float, reps: int = 20, rounds: int = 3):
     xs = [torch.rand(b, 2 * d, device=dev) for _ in range(4)]
     ys = [torch.empty_like(xs[0]) for _ in range(4)]
     tab = ops.table(_lib.KIND_C2C, n, dev)
-    args = [(xs[i].data_ptr(), d, ys[i].data_ptr(), d, n, b, -1, tab.data_ptr(), sid()) for i in range(4)]
+    args = [(xs[i].data_ptr(), d, ys[i].data_ptr(), d, n, b, -1, tab.data_ptr(), None, sid()) for i in range(4)]
     out["c2c_rows_n4096_b4096_dist4160"] = row(
         timed(lambda i: _lib.check(lib.bfly_c2c_rows(*args[i % 4]), "rows")), 16 * n * b, b)
     args = [(xs[i].data_ptr(), ys[i].data_ptr(), n, b, 1, 1.0 / n, tab.data_ptr(), None, sid()) for i in range(4)]

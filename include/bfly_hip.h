@@ -192,15 +192,19 @@ BFLY_API int bfly_c2c_columns(const float* in, float* out, uint32_t n, uint32_t 
 /* Batched 1-D C2C over padded rows: transform r reads n complex values at
  * in + 2 r idist floats and writes n at out + 2 r odist floats (idist, odist
  * >= n complex values; element stride 1).  The gaps between rows are neither
- * read nor written.  n a power of two in 16..16384, `table` the device copy
- * of bfly_table_fill(BFLY_KIND_C2C, n), no workspace; in and out 8-byte
- * aligned, their spans [p, p + 8 ((batch-1) dist + n)) disjoint.  Runs the
- * LDS-twiddle streaming kernel, so with idist = odist = n it equals
- * bfly_c2c_ex(..., BFLY_VARIANT_STREAM_LT, ...) bit for bit.  No counterpart
+ * read nor written.  n a power of two in 16..65536, `table` the device copy
+ * of bfly_table_fill(BFLY_KIND_C2C, n); in and out 8-byte aligned, their
+ * spans [p, p + 8 ((batch-1) dist + n)) disjoint.  Up to 16384 runs the
+ * LDS-twiddle streaming kernel (no workspace; with idist = odist = n equal
+ * to bfly_c2c_ex(..., BFLY_VARIANT_STREAM_LT, ...) bit for bit); 32768 and
+ * 65536 run the four-step chain, whose column pass reads the padded input
+ * and row pass writes the padded output, through `work` of
+ * bfly_workspace_bytes(BFLY_KIND_C2C, n, batch) bytes (with idist = odist
+ * = n equal to bfly_c2c bit for bit).  No counterpart
  * in the reference: non-contiguous contracts are future work there
  * (config/test_matrix.json:94-99, 106 "fft2d-and-strides"). */
 BFLY_API int bfly_c2c_rows(const float* in, uint64_t idist, float* out, uint64_t odist, uint32_t n, uint32_t batch,
-                           int sign, const float* table, void* stream);
+                           int sign, const float* table, void* work, void* stream);
 BFLY_API size_t bfly_fft2d_table_floats(uint32_t n1);
 BFLY_API int bfly_fft2d_table_fill(uint32_t n1, float* host_table);
 BFLY_API size_t bfly_fft2d_workspace_bytes(uint32_t n1, uint32_t n2, uint32_t batch);

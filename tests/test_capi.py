@@ -204,23 +204,29 @@ def test_padded_rows_guards_through_stub():
     s = bfly.stub()
     buf = np.zeros(1 << 16, np.float32)
     p = buf.ctypes.data
-    i, o, t = p, p + (1 << 30), p + (3 << 30)
-    assert s.bfly_c2c_rows(i, 4100, o, 4096, 4096, 16, -1, t, None) == 0
-    assert s.bfly_c2c_rows(i, 20000, o, 16384, 16384, 3, 1, t, None) == 0
-    assert s.bfly_c2c_rows(i, 1, o, 1, 64, 1, -1, t, None) == bfly.ERR_SHAPE  # one row still needs dist >= n
-    assert s.bfly_c2c_rows(i, 4095, o, 4096, 4096, 16, -1, t, None) == bfly.ERR_SHAPE  # rows would overlap
-    assert s.bfly_c2c_rows(i, 4096, o, 4095, 4096, 16, -1, t, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c_rows(i, 32768, o, 32768, 32768, 1, -1, t, None) == bfly.ERR_SHAPE  # four-step lengths
-    assert s.bfly_c2c_rows(i, 48, o, 48, 48, 1, -1, t, None) == bfly.ERR_SHAPE  # not a power of two
-    assert s.bfly_c2c_rows(i + 4, 64, o, 64, 64, 1, -1, t, None) == bfly.ERR_SHAPE  # not 8-byte aligned
-    assert s.bfly_c2c_rows(i, 64, o, 64, 64, 0, -1, t, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c_rows(i, 64, o, 64, 64, 1, 0, t, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c_rows(i, 64, o, 64, 64, 1, -1, None, None) == _lib.ERR_TABLE
-    assert s.bfly_c2c_rows(i, 1 << 58, o, 64, 64, 2, -1, t, None) == bfly.ERR_SHAPE  # extent past 2^58
+    i, o, w, t = p, p + (1 << 30), p + (2 << 30), p + (3 << 30)
+    assert s.bfly_c2c_rows(i, 4100, o, 4096, 4096, 16, -1, t, None, None) == 0
+    assert s.bfly_c2c_rows(i, 20000, o, 16384, 16384, 3, 1, t, None, None) == 0
+    assert s.bfly_c2c_rows(i, 1, o, 1, 64, 1, -1, t, None, None) == bfly.ERR_SHAPE  # one row still needs dist >= n
+    assert s.bfly_c2c_rows(i, 4095, o, 4096, 4096, 16, -1, t, None, None) == bfly.ERR_SHAPE  # rows would overlap
+    assert s.bfly_c2c_rows(i, 4096, o, 4095, 4096, 16, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_rows(i, 48, o, 48, 48, 1, -1, t, None, None) == bfly.ERR_SHAPE  # not a power of two
+    assert s.bfly_c2c_rows(i, 131072, o, 131072, 131072, 1, -1, t, w, None) == bfly.ERR_SHAPE  # past 65536
+    assert s.bfly_c2c_rows(i + 4, 64, o, 64, 64, 1, -1, t, None, None) == bfly.ERR_SHAPE  # not 8-byte aligned
+    assert s.bfly_c2c_rows(i, 64, o, 64, 64, 0, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_rows(i, 64, o, 64, 64, 1, 0, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_rows(i, 64, o, 64, 64, 1, -1, None, None, None) == _lib.ERR_TABLE
+    assert s.bfly_c2c_rows(i, 1 << 58, o, 64, 64, 2, -1, t, None, None) == bfly.ERR_SHAPE  # extent past 2^58
     # spans: 8 ((b-1) dist + n) bytes; n=64, b=4, dist=100 -> 2912
-    assert s.bfly_c2c_rows(i, 100, i + 2912, 100, 64, 4, -1, t, None) == 0
-    assert s.bfly_c2c_rows(i, 100, i + 2904, 100, 64, 4, -1, t, None) == bfly.ERR_SHAPE
-    assert s.bfly_c2c_rows(i + 2912, 100, i, 100, 64, 4, -1, t, None) == 0
+    assert s.bfly_c2c_rows(i, 100, i + 2912, 100, 64, 4, -1, t, None, None) == 0
+    assert s.bfly_c2c_rows(i, 100, i + 2904, 100, 64, 4, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_rows(i + 2912, 100, i, 100, 64, 4, -1, t, None, None) == 0
+    # four-step lengths need a contiguous workspace of 8 n batch bytes
+    assert s.bfly_c2c_rows(i, 70000, o, 65536, 65536, 2, -1, t, None, None) == bfly.ERR_SHAPE
+    assert s.bfly_c2c_rows(i, 70000, o, 65536, 65536, 2, -1, t, w, None) == 0
+    assert s.bfly_c2c_rows(i, 40000, o, 32768, 32768, 3, 1, t, w, None) == 0
+    assert s.bfly_c2c_rows(i, 70000, o, 65536, 65536, 2, -1, t, o + 8 * 65536, None) == bfly.ERR_SHAPE  # in out
+    assert s.bfly_c2c_rows(i, 70000, o, 65536, 65536, 2, -1, t, i + 8 * 70000 - 8, None) == bfly.ERR_SHAPE
 
 
 @pytest.mark.parametrize("norm", [None, "backward", "ortho", "forward"])

@@ -60,7 +60,8 @@ def test_graph_replays_equal_eager_calls(dev):
               lib.bfly_c2c_columns(xc.data_ptr(), o[5].data_ptr(), 1024, 33, 2, 1, tcc.data_ptr(), wc.data_ptr(),
                                    sid),
               lib.bfly_c2c_scaled(x1.data_ptr(), o[6].data_ptr(), 4096, 64, 1, 1.0 / 4096, t1.data_ptr(), None, sid),
-              lib.bfly_c2c_rows(xp.data_ptr(), 1032, o[7].data_ptr(), 1024, 1024, 16, -1, tp.data_ptr(), sid)]
+              lib.bfly_c2c_rows(xp.data_ptr(), 1032, o[7].data_ptr(), 1024, 1024, 16, -1, tp.data_ptr(), None,
+                                sid)]
         assert rc == [0] * 8, rc
 
     torch.cuda.synchronize()

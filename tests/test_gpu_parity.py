@@ -242,12 +242,13 @@ def test_c2c_scaled_equals_its_variant_times_scale_bit_for_bit(n, b):
 
 
 @pytest.mark.parametrize("n,b,ipad,opad", [(16, 5, 1, 3), (256, 97, 8, 0), (4096, 40, 3, 64), (8192, 6, 1, 1),
-                                           (16384, 3, 2, 0), (1024, 1, 5, 5)])
+                                           (16384, 3, 2, 0), (1024, 1, 5, 5), (32768, 3, 4, 2), (65536, 2, 1, 0)])
 def test_c2c_rows_padded_equals_contiguous_bit_for_bit(n, b, ipad, opad):
     # bfly_c2c_rows over rows n + ipad apart into rows n + opad apart: the
-    # LDS-twiddle streaming kernel with row distances, so it equals the
-    # contiguous STREAM_LT call bit for bit.  NaN in the input gaps would
-    # reach the result if a gap were read; the output gaps keep their fill.
+    # LDS-twiddle streaming kernel (up to 16384) or the four-step chain
+    # (32768 / 65536) with row distances, so it equals the contiguous call
+    # of the same kernels bit for bit.  NaN in the input gaps would reach
+    # the result if a gap were read; the output gaps keep their fill.
     import torch
     from bfly import _lib
     ops, lib = _ops(), _lib.hip()
@@ -262,8 +263,10 @@ def test_c2c_rows_padded_equals_contiguous_bit_for_bit(n, b, ipad, opad):
         obuf = torch.full((b, n + opad), complex(7.0, -7.0), dtype=torch.complex64, device="cuda")
         ops.c2c_rows(ibuf[:, :n], sign, out=obuf[:, :n])
         ref = torch.empty_like(xt)
-        assert lib.bfly_c2c_ex(xt.data_ptr(), ref.data_ptr(), n, b, sign, _lib.VARIANT_STREAM_LT, tab.data_ptr(),
-                               None, st) == 0
+        work = ops.workspace(_lib.KIND_C2C, n, b, xt.device) if n > 16384 else None
+        variant = _lib.VARIANT_STREAM_LT if n <= 16384 else _lib.VARIANT_FOUR_STEP
+        assert lib.bfly_c2c_ex(xt.data_ptr(), ref.data_ptr(), n, b, sign, variant, tab.data_ptr(),
+                               work.data_ptr() if work is not None else None, st) == 0
         torch.cuda.synchronize()
         got = torch.view_as_real(obuf[:, :n]).cpu()
         assert torch.equal(got, torch.view_as_real(ref).cpu()), (n, b, sign)
