@@ -229,6 +229,19 @@ def test_padded_rows_guards_through_stub():
     assert s.bfly_c2c_rows(i, 70000, o, 65536, 65536, 2, -1, t, i + 8 * 70000 - 8, None) == bfly.ERR_SHAPE
 
 
+def test_rows_and_scale_wrappers_refuse_bad_arguments_on_the_host():
+    import torch
+    from bfly import ops
+    with pytest.raises(ValueError):  # host tensor: refused before any device call
+        ops.c2c_rows(torch.zeros(2, 8, dtype=torch.complex64))
+    with pytest.raises(ValueError):
+        ops.c2c(torch.zeros(2, 8, dtype=torch.complex64), -1, norm="ortho", scale=0.5)
+    assert ops.norm_scale(64, -1, None) is None
+    assert ops.norm_scale(64, 1, "backward") == 1.0 / 64
+    assert ops.norm_scale(64, -1, "forward") == 1.0 / 64
+    assert ops.norm_scale(64, 1, "ortho") == ops.norm_scale(64, -1, "ortho") == 0.125
+
+
 @pytest.mark.parametrize("norm", [None, "backward", "ortho", "forward"])
 def test_norm_scale_follows_numpy(norm):
     from bfly import ops
