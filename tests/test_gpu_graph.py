@@ -2,7 +2,7 @@
 
 One graph holds a C2C (streaming LDS-twiddle kernel), a four-step long C2C,
 an R2C, a C2R, a 2-D C2C, a strided column transform, a normalised inverse
-and a padded-row C2C.  It is replayed
+and padded-row C2Cs (one-pass and four-step).  It is replayed
 several times with the program's own eager work between the replays (new
 inputs written, the same calls run eagerly into separate outputs, full
 synchronisations), and every replay must equal the eager results bit for
@@ -36,9 +36,11 @@ def test_graph_replays_equal_eager_calls(dev):
     x2 = rnd(3, 512, 2 * 256)
     xc = rnd(2, 1024, 2 * 33)
     xp = rnd(16, 2 * (1024 + 8))  # padded rows: 1024 values every 1032
-    ins = [x1, xl, xr, hr, x2, xc, xp]
+    xq = rnd(2, 2 * (32768 + 16))  # padded rows through the four-step chain
+    ins = [x1, xl, xr, hr, x2, xc, xp, xq]
     outs = [torch.empty_like(x1), torch.empty_like(xl), torch.empty_like(hr), torch.empty_like(xr),
-            torch.empty_like(x2), torch.empty_like(xc), torch.empty_like(x1), torch.empty(16, 2 * 1024, device=dev)]
+            torch.empty_like(x2), torch.empty_like(xc), torch.empty_like(x1), torch.empty(16, 2 * 1024, device=dev),
+            torch.empty(2, 2 * 32768, device=dev)]
     eager = [torch.empty_like(o) for o in outs]
     t1 = ops.table(_lib.KIND_C2C, 4096, dev)
     tl = ops.table(_lib.KIND_C2C, 65536, dev)
@@ -49,6 +51,8 @@ def test_graph_replays_equal_eager_calls(dev):
     tcc = ops.table2d_cols(1024, dev)
     wc = torch.empty(2 * xc.numel(), device=dev)
     tp = ops.table(_lib.KIND_C2C, 1024, dev)
+    tq = ops.table(_lib.KIND_C2C, 32768, dev)
+    wq = ops.workspace(_lib.KIND_C2C, 32768, 2, dev)
 
     def calls(o):
         rc = [lib.bfly_c2c(x1.data_ptr(), o[0].data_ptr(), 4096, 64, -1, t1.data_ptr(), None, sid),
@@ -61,8 +65,10 @@ def test_graph_replays_equal_eager_calls(dev):
                                    sid),
               lib.bfly_c2c_scaled(x1.data_ptr(), o[6].data_ptr(), 4096, 64, 1, 1.0 / 4096, t1.data_ptr(), None, sid),
               lib.bfly_c2c_rows(xp.data_ptr(), 1032, o[7].data_ptr(), 1024, 1024, 16, -1, tp.data_ptr(), None,
-                                sid)]
-        assert rc == [0] * 8, rc
+                                sid),
+              lib.bfly_c2c_rows(xq.data_ptr(), 32784, o[8].data_ptr(), 32768, 32768, 2, 1, tq.data_ptr(),
+                                wq.data_ptr(), sid)]
+        assert rc == [0] * 9, rc
 
     torch.cuda.synchronize()
     with torch.cuda.stream(s):
