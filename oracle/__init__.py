@@ -140,12 +140,6 @@ def ref():
         fn("ref_fold_d_for", u32, u32, u32, u32)
         fn("ref_row_fft_ub_bytes", C.c_uint64, C.c_uint64, C.c_uint64, C.c_uint64)
         fn("ref_resolve_row_fft", C.c_int, u32, u32, u32, u32, C.POINTER(u32), C.POINTER(u32))
-        fn("ref_gen_twiddles", None, u32, _f32p, _f32p)
-        fn("ref_gen_bit_reverse", None, u32, _u32p)
-        fn("ref_gen_bit_reverse_plane", None, u32, _u32p)
-        fn("ref_gen_transpose_plane", None, u32, _u32p)
-        fn("ref_gen_interleave", None, u32, u32, _u32p)
-        fn("ref_estimate_us", C.c_double, u32, u32, C.c_int, C.c_int, C.c_int)
         fn("ref_default_mapping", None, u32, u32, C.POINTER(RefMapping))
         fn("ref_query_lowering", None, C.POINTER(RefMapping), C.c_char_p, C.c_char_p,
            C.c_char_p, C.POINTER(RefLowering))

@@ -2,12 +2,16 @@
 
 Each restated function is checked against at least one thing that is not its
 own formula:
-  * the reference's own code compiled from its sources (oracle/_ref), bit for bit;
-  * golden vectors written by that build (tests/golden/reference_vectors.npz);
+  * the reference's own code compiled from its sources (oracle/_ref: reference.cpp
+    and the header-only fft_k.hpp / long_fft_ub.h / descriptors.hpp, plain g++,
+    no stand-in headers), bit for bit;
+  * golden vectors written by that build and by the reference's Python input
+    twin (tests/golden/reference_vectors.npz, tests/golden/make_goldens.py);
   * goldens the reference's tests hold (tests/test_limits.cpp,
     tests/test_runtime_guards.cpp, tests/test_descriptors.py);
   * mathematics: direct O(N^2) DFT, numpy pocketfft in FP64, closed forms
-    (impulse, constant, tone), Parseval, linearity, round trips.
+    (impulse, constant, tone), Parseval, linearity, round trips, the FFT the
+    Generator tables drive, Marsaglia's xorshift32 known answer.
 """
 import ctypes as C
 from pathlib import Path
@@ -162,6 +166,26 @@ def test_pattern_golden():
     assert np.array_equal(got, GOLD["pattern_0_4096"])
 
 
+def test_pattern_closed_form():
+    """patternAt (src/framework/reference.cpp:10-12): the index is a size_t, so
+    i * 2654435761 wraps modulo 2^64 (not 2^32); the residue mod 1000 is
+    divided by 1000 in fp32 and 0.5 subtracted in fp32.  Python integers give
+    the 64-bit product exactly; indices above 2^32 tell the two wraps apart."""
+    idx = list(range(5000)) + [2**32 - 1, 2**32, 2**32 + 12345, 2**40 + 7, 2**63 + 99]
+    want = [np.float32(np.float32(((i * 2654435761) % 2**64) % 1000) / np.float32(1000.0)) - np.float32(0.5)
+            for i in idx]
+    got = [oracle.lib().orc_pattern_at(i) for i in idx]
+    assert np.array_equal(np.array(got, np.float32), np.array(want, np.float32))
+    wrap32 = np.float32(((2**32 + 12345) * 2654435761 % 2**32) % 1000) / np.float32(1000) - np.float32(0.5)
+    assert got[idx.index(2**32 + 12345)] != wrap32
+
+
+@needs_ref
+def test_pattern_live():
+    for i in list(range(0, 70000, 7)) + [2**32 + 5, 2**40 + 7]:
+        assert oracle.lib().orc_pattern_at(i) == REF.ref_pattern_at(i)
+
+
 def test_integer_rules_golden():
     L = oracle.lib()
     ns = GOLD["rule_n"]
@@ -222,36 +246,96 @@ def test_resolve_row_fft_live():
                     assert (x, a.value, b.value) == (y, c.value, e.value)
 
 
-@pytest.mark.parametrize("n", [64, 256, 4096])
-def test_generator_tables_golden(n):
+def _np_bitrev(n):
+    """Bit reversal by reversing each index's binary string."""
+    bits = n.bit_length() - 1
+    if bits == 0:
+        return np.zeros(1, np.int64)
+    return np.array([int(format(i, f"0{bits}b")[::-1], 2) for i in range(n)], np.int64)
+
+
+def _tables(n):
     L = oracle.lib()
     twr, twi = np.empty(n + 16, np.float32), np.empty(n + 16, np.float32)
     L.orc_gen_twiddles(n, twr, twi)
-    assert np.array_equal(twr, GOLD[f"gen_twr_{n}"]) and np.array_equal(twi, GOLD[f"gen_twi_{n}"])
+    out = {"twr": twr, "twi": twi}
     for name, fn in (("bitrev", L.orc_gen_bit_reverse), ("plane", L.orc_gen_bit_reverse_plane),
                      ("transpose", L.orc_gen_transpose_plane)):
         d = np.empty(n, np.uint32)
         fn(n, d)
-        assert np.array_equal(d, GOLD[f"gen_{name}_{n}"]), name
-    for D in (1, 4):
+        out[name] = d.astype(np.int64)
+    return out
+
+
+@pytest.mark.parametrize("n", [8, 16, 64, 256, 1024, 4096, 65536])
+def test_generator_tables_drive_a_radix2_fft(n):
+    """genTwiddles + genBitReverse (src/framework/butterfly.cpp:96-123) are the
+    tables of a radix-2 DIT: gathering by the byte offsets / 8 and running the
+    h = 1, 2, ..., n/2 butterfly sweep with stage slot s(h) = sum of align8(h')
+    over h' < h gives the DFT (numpy pocketfft, FP64).  A wrong sign, slot
+    alignment, stage order or permutation fails here."""
+    t = _tables(n)
+    rng = np.random.default_rng(n)
+    x = rng.uniform(-1, 1, n) + 1j * rng.uniform(-1, 1, n)
+    a = x[t["bitrev"] // 2].astype(np.complex128)
+    slot, h = 0, 1
+    while 2 * h <= n:
+        w = t["twr"][slot:slot + h].astype(np.float64) + 1j * t["twi"][slot:slot + h].astype(np.float64)
+        blk = a.reshape(-1, 2 * h)
+        u, v = blk[:, :h], blk[:, h:] * w
+        a = np.concatenate([u + v, u - v], axis=1).reshape(-1)
+        slot += (h + 7) // 8 * 8
+        h *= 2
+    want = np.fft.fft(x)
+    assert np.abs(a - want).max() <= 1e-6 * np.abs(want).max() * np.log2(n)
+    assert slot <= n + 16 and not t["twr"][slot:].any() and not t["twi"][slot:].any()
+
+
+@pytest.mark.parametrize("n", [8, 64, 256, 4096, 65536])
+def test_generator_tables_closed_forms(n):
+    """Every table against a numpy construction of its meaning, with K from the
+    oracle's planeKFor (pinned below by the reference's goldens):
+      twiddles: FP64 e^{-i pi j/h} rounded once to fp32, per align8(h) slot;
+      bitrev:   2 * (index with its binary string reversed);
+      plane:    the bit-reversed order viewed as [rows][K], transposed into K
+                planes of rows, 8-byte offsets;
+      transpose: [K][rows] -> [rows][K] index map, 4-byte offsets;
+      interleave: gather of a planar [re plane | im plane] of D rows into
+                interleaved [D][n][re, im], 4-byte offsets."""
+    L = oracle.lib()
+    t = _tables(n)
+    K = L.orc_plane_k_for(n)
+    rows = n // K
+    slots = []
+    h = 1
+    while 2 * h <= n:
+        slots += [np.exp(-1j * np.pi * np.arange(h) / h), np.zeros((h + 7) // 8 * 8 - h)]
+        h *= 2
+    want = np.concatenate(slots + [np.zeros(n + 16 - sum(len(s) for s in slots))])
+    assert np.array_equal(t["twr"], want.real.astype(np.float32))
+    assert np.array_equal(t["twi"], want.imag.astype(np.float32))
+    br = _np_bitrev(n)
+    assert np.array_equal(t["bitrev"], 2 * br)
+    assert np.array_equal(t["plane"], 8 * br[np.arange(n).reshape(rows, K).T.ravel()])
+    assert np.array_equal(t["transpose"], 4 * np.arange(n).reshape(K, rows).T.ravel())
+    rng = np.random.default_rng(7)
+    for D in (1, 2, 4):
         d = np.empty(2 * n * D, np.uint32)
         L.orc_gen_interleave(n, D, d)
-        assert np.array_equal(d, GOLD[f"gen_interleave_{n}_{D}"])
+        re = rng.normal(size=(D, n)).astype(np.float32)
+        im = rng.normal(size=(D, n)).astype(np.float32)
+        planar = np.concatenate([re.ravel(), im.ravel()]).view(np.uint8)
+        gathered = np.stack([planar[d.astype(np.int64) + b] for b in range(4)], axis=1).copy().view(np.float32)
+        assert np.array_equal(gathered.ravel(), np.stack([re, im], axis=-1).ravel())
 
 
 def test_generator_permutation_properties():
-    L = oracle.lib()
     for n in (64, 1024, 4096):
-        br = np.empty(n, np.uint32)
-        L.orc_gen_bit_reverse(n, br)
-        r = br // 2
+        t = _tables(n)
+        r = t["bitrev"] // 2
         assert sorted(r) == list(range(n)) and np.array_equal(r[r], np.arange(n))  # involution
-        plane = np.empty(n, np.uint32)
-        L.orc_gen_bit_reverse_plane(n, plane)
-        assert sorted(plane // 8) == list(range(n))
-        tp = np.empty(n, np.uint32)
-        L.orc_gen_transpose_plane(n, tp)
-        assert sorted(tp // 4) == list(range(n))
+        assert sorted(t["plane"] // 8) == list(range(n))
+        assert sorted(t["transpose"] // 4) == list(range(n))
 
 
 # ------------------------------------------------------------ error metrics
@@ -296,24 +380,66 @@ def test_rel_l2_metric():
     assert np.isinf(oracle.rel_l2_rows(got, ref))
 
 
-def test_input_generators_match_reference_twin():
-    # scripts/bench_native_npu.py:27-37 documents the appHash twin; check the
-    # hash against its definition (xorshift32, two rounds) and value ranges.
-    def app_hash(b, k):
-        x = (b * 2654435761 + k * 40503) & 0xFFFFFFFF
-        for _ in range(2):
-            x ^= (x << 13) & 0xFFFFFFFF
-            x ^= x >> 17
-            x ^= (x << 5) & 0xFFFFFFFF
-        return x
+def test_app_hash_known_answer():
+    """appHash (src/host/fft_check.cpp:63-68) is two steps of Marsaglia's
+    xorshift32 (13, 17, 5) from the seed b*2654435761 + k*40503 mod 2^32.
+    Marsaglia's published sequence from y = 2463534242 starts 723471715,
+    2497366906 (Xorshift RNGs, J. Stat. Softw. 8(14), 2003); b = 296369474
+    is the one b whose seed is 2463534242 at k = 0, so appHash(b, 0) is the
+    sequence's second value.  The k multiplier is pinned by the reference
+    twin's values below."""
     L = oracle.lib()
-    for b, k in ((0, 0), (1, 2), (4095, 4095), (123456, 987654)):
-        assert L.orc_app_hash(b, k) == app_hash(b, k)
+    assert (296369474 * 2654435761) % 2**32 == 2463534242
+    assert L.orc_app_hash(296369474, 0) == 2497366906
+
+
+def test_app_hash_matches_reference_twin():
+    # scripts/bench_native_npu.py:27-37 (`_xh`), run by tests/golden/make_goldens.py
+    L = oracle.lib()
+    for i, b in enumerate(GOLD["hash_b"]):
+        for j, k in enumerate(GOLD["hash_k"]):
+            assert L.orc_app_hash(int(b), int(k)) == GOLD["hash_out"][i, j], (b, k)
+
+
+@pytest.mark.parametrize("mode", ["ofdm", "radar", "dl"])
+def test_app_inputs_match_reference_twin(mode):
+    """ofdm / radar / dl (src/host/fft_check.cpp:118-159) against the vectors
+    the reference's own Python twin wrote (scripts/bench_native_npu.py:40-84,
+    stated bit-identical to fft_check): bit for bit for ofdm and dl (table
+    lookups); radar sums FP64 cos/sin, so one fp32 ulp is allowed where the
+    two libm evaluations round differently."""
+    for n, b in ((64, 3), (256, 14), (2048, 5)):
+        got = oracle.app_input(n, b, mode)
+        want = GOLD[f"app_{mode}_{n}_{b}"]
+        if mode == "radar":
+            ulps = np.abs(got.view(np.int32).astype(np.int64) - want.view(np.int32).astype(np.int64))
+            assert ulps.max() <= 1 and (ulps == 0).mean() > 0.99, (n, b)
+        else:
+            assert np.array_equal(got, want), (n, b)
+
+
+def test_app_input_properties():
+    """Structure each application input must have, independent of the hash:
+    ofdm guard bands and DC are zero, pilots every 12 bins are unit QPSK, the
+    rest 16-QAM scaled 1/sqrt(10); dl is constant over blocks of 8 with a
+    4-PSK phase; radar is bounded by the sum of its target amplitudes / 4."""
+    n, b = 1024, 6
+    x = oracle.app_input(n, b, "ofdm").reshape(b, n, 2)
+    k = np.arange(n)
+    guard = (k < n // 16) | (k >= n - n // 16)
+    assert not x[:, guard].any()
+    pil = x[:, (~guard) & (k % 12 == 0)]
+    assert set(np.unique(np.abs(pil))) == {1.0}
+    data = x[:, (~guard) & (k % 12 != 0)] * np.float32(np.sqrt(10))
+    assert set(np.round(np.unique(data), 5)) == {-3.0, -1.0, 1.0, 3.0}
+    d = oracle.app_input(n, b, "dl").reshape(b, n // 8, 8, 2)
+    assert np.all(d == d[:, :, :1, :]) and np.all((d[..., 0] == 0) | (d[..., 1] == 0))
+    r = oracle.app_input(n, b, "radar")
+    assert np.abs(r).max() <= 0.25 * 1.875 + 1e-6
     h = oracle.half_input(64, 3)
     assert np.all(h[:, 65] == 0) and np.all(np.abs(h) <= 1)
     for mode in oracle.NUMERICAL_MODES:
-        x = oracle.numerical_input(64, 2, mode)
-        assert np.all(np.isfinite(x))
+        assert np.all(np.isfinite(oracle.numerical_input(64, 2, mode)))
 
 
 def test_sin_input_sequence():
