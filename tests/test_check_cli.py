@@ -1,5 +1,5 @@
 """CPU: the bfly_check program (ascend-fft_amd/csrc/check/) against the C
-ABI's CPU stub, and the report parser bfly.scopes.
+ABI's CPU stub, and the report reader bfly.report.
 
 The stub runs every guard but computes nothing, which makes it the planted
 mistake for the program's own checks: every transform must FAIL and a
@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import oracle
-from bfly import scopes
+from bfly import report
 
 ROOT = Path(__file__).resolve().parents[1]
 PKG = ROOT / "ascend-fft_amd"
@@ -110,29 +110,29 @@ def test_stale_outputs_are_flagged(cli):
     for n in (1024, 8192, 65536):
         r = run(cli, n, 2, 1, AB_INPUT_SEQ="sin,random-seeded,sin")
         assert r.returncode == 1, r.stdout
-        rep = scopes.parse_report(r.stdout)
-        assert [s[3] for s in rep["seq"]] == [False, True, True], r.stdout
+        rep = report.Report.parse(r.stdout)
+        assert [s.stale for s in rep.seq] == [False, True, True], r.stdout
         assert "stale-check on -> FAIL" in r.stdout
         # identical consecutive inputs are not stale
         r = run(cli, n, 2, 1, AB_INPUT_SEQ="zero,zero")
-        assert [s[3] for s in scopes.parse_report(r.stdout)["seq"]] == [False, False]
+        assert [s.stale for s in report.Report.parse(r.stdout).seq] == [False, False]
 
 
 def test_report_lines_follow_the_vocabulary(cli):
     r = run(cli, 1024, 4, 3)
-    rep = scopes.parse_report(r.stdout)
-    assert rep["verdict"] == "FAIL" and rep["launches"] == 1 and rep["boundary_impl"] is None
-    f = rep["scopes"]
-    assert f["reps"] == 3 and f["h2d"] is None and f["d2h"] is None and rep["segments"] is None
-    assert scopes.infer_mode(f) == "short"
-    for n, b, impl, segs in ((8192, 300, "single", {"fft1"}), (16384, 2, "fused2", {"fft1", "fft2"}),
-                             (65536, 1, "fused2", {"fft1", "fft2"})):
-        rep = scopes.parse_report(run(cli, n, b, 2).stdout)
-        assert rep["boundary_impl"] == impl and set(rep["segments"]) == segs, (n, b)
-        assert rep["scopes"]["h2d"] is not None and rep["scopes"]["d2h"] is not None
-        assert scopes.infer_mode(rep["scopes"]) == "long_device"
-    rep = scopes.parse_report(run(cli, 8192, 3, 2, AB_VARIANT="four_step").stdout)
-    assert rep["boundary_impl"] == "fused2" and rep["launches"] == 2
+    rep = report.Report.parse(r.stdout)
+    assert rep.verdict == "FAIL" and rep.launches == 1 and rep.chain is None
+    f = rep.scopes
+    assert f.reps == 3 and f.h2d is None and f.d2h is None and rep.segments is None
+    assert rep.mode == "resident"
+    for n, b, chain, segs in ((8192, 300, "single", {"fft1"}), (16384, 2, "fused2", {"fft1", "fft2"}),
+                              (65536, 1, "fused2", {"fft1", "fft2"})):
+        rep = report.Report.parse(run(cli, n, b, 2).stdout)
+        assert rep.chain == chain and set(rep.segments) == segs, (n, b)
+        assert rep.scopes.h2d is not None and rep.scopes.d2h is not None
+        assert rep.mode == "staged"
+    rep = report.Report.parse(run(cli, 8192, 3, 2, AB_VARIANT="four_step").stdout)
+    assert rep.chain == "fused2" and rep.launches == 2
     # the lowering descriptor of the chain (include/butterfly/descriptors.hpp) and its split
     out = run(cli, 65536, 2, 1, AB_DESC=1).stdout
     assert "lowering: launches=2 gm_boundaries=1 modeled_payload_gm=4194304 impl=fused2" in out
@@ -148,59 +148,74 @@ def test_report_lines_follow_the_vocabulary(cli):
     assert "E2E transfers: in=1 out=1 boundary=0 per_execution" in r.stdout
 
 
-# ---------------------------------------------------------------- parser
-DEVICE = ("scopes: plan_setup=812.5 us first_use=950.0 us host_end_to_end mean=4100.0 min=3900.0 us "
+# ---------------------------------------------------------------- reader
+STAGED = ("scopes: plan_setup=812.5 us first_use=950.0 us host_end_to_end mean=4100.0 min=3900.0 us "
           "h2d=1500.0 us device_chain=210.0 us d2h=1450.0 us reps=5")
-SHORT = ("scopes: plan_setup=300.0 us first_use=400.0 us host_end_to_end mean=60.0 min=55.0 us "
-         "h2d=NA device_chain=47.5 us d2h=NA reps=20")
+RESIDENT = ("scopes: plan_setup=300.0 us first_use=400.0 us host_end_to_end mean=60.0 min=55.0 us "
+            "h2d=NA device_chain=47.5 us d2h=NA reps=20")
 
 
-def test_parse_scopes_fields_and_na():
-    f = scopes.parse_scopes(DEVICE)
-    assert sorted(f) == sorted(scopes.FIELDS)
-    assert (f["h2d"], f["device_chain"], f["d2h"], f["e2e_mean"], f["e2e_min"], f["reps"]) == \
-        (1500.0, 210.0, 1450.0, 4100.0, 3900.0, 5)
-    s = scopes.parse_scopes(SHORT)
-    assert s["h2d"] is None and s["d2h"] is None and s["device_chain"] == 47.5
-    for bad in (DEVICE.replace(" d2h=1450.0 us", ""), "PASS",
+def test_token_reader():
+    assert report.read_pairs("a=1 us lab b=2 c=NA us d=3") == {"a": "1", "lab.b": "2", "lab.c": None, "d": "3"}
+    for bad in ("a=1 a=2", "x y a=1", "=3", "a="):
+        with pytest.raises(report.ReportError):
+            report.read_pairs(bad)
+
+
+def test_scopes_line_fields_and_na():
+    f = report.Scopes.parse(STAGED)
+    assert (f.h2d, f.device_chain, f.d2h, f.e2e_mean, f.e2e_min, f.reps) == (1500.0, 210.0, 1450.0, 4100.0, 3900.0, 5)
+    assert f.mode == "staged"
+    s = report.Scopes.parse(RESIDENT)
+    assert s.h2d is None and s.d2h is None and s.device_chain == 47.5 and s.mode == "resident"
+    for bad in (STAGED.replace(" d2h=1450.0 us", ""), "PASS", STAGED.replace("reps=5", "reps=five"),
                 "scopes: plan_setup=1.0 us first_use=2.0 us host_end_to_end mean=3.0 min=2.0 us "
                 "device_only=1.0 us reps=1"):
-        with pytest.raises(ValueError):
-            scopes.parse_scopes(bad)
+        with pytest.raises(report.ReportError):
+            report.Scopes.parse(bad)
 
 
-def test_validate_scopes_rules():
-    assert scopes.validate_scopes(scopes.parse_scopes(SHORT), "short") == []
-    assert scopes.validate_scopes(scopes.parse_scopes(DEVICE), "long_device") == []
-    # the short path must not report transfers; long chains must
-    assert any("h2d=NA" in p for p in scopes.validate_scopes(scopes.parse_scopes(DEVICE), "short"))
-    assert any("h2d must be measured" in p for p in
-               scopes.validate_scopes(scopes.parse_scopes(SHORT), "long_device"))
-    # transfers folded into the chain overrun the wall clock
-    bad = scopes.parse_scopes(DEVICE.replace("device_chain=210.0", "device_chain=9000.0"))
-    assert any("transfers are not allowed" in p for p in scopes.validate_scopes(bad, "long_device"))
-    assert scopes.validate_scopes(scopes.parse_scopes(SHORT), "sideways")[0].startswith("unknown mode")
+def test_scope_rules():
+    staged, resident = report.Scopes.parse(STAGED), report.Scopes.parse(RESIDENT)
+    assert report.scope_problems(resident) == [] and report.scope_problems(staged) == []
+    # a resident run must not time copies; a staged one must
+    assert any("want NA" in p for p in report.scope_problems(staged, "resident"))
+    assert any("must time both copies" in p for p in report.scope_problems(resident, "staged"))
+    # copies counted inside the chain overrun the wall clock
+    bad = report.Scopes.parse(STAGED.replace("device_chain=210.0", "device_chain=9000.0"))
+    assert any("copies were counted inside device_chain" in p for p in report.scope_problems(bad))
+    # the tolerance is the caller's
+    assert report.scope_problems(bad, tol=report.Tolerance(wall_us=1e5)) == []
+    assert report.scope_problems(resident, "sideways")[0].startswith("mode 'sideways'")
 
 
-def test_segments_and_boundary_impl():
-    assert scopes.parse_segments("segments: NA") is None
-    seg = scopes.parse_segments("segments: fft1=120.0 us fft2=90.0 us")
+def test_segments_and_chain_forms():
+    assert report.parse_segments("segments: NA") is None
+    seg = report.parse_segments("segments: fft1=120.0 us fft2=90.0 us")
     assert seg == {"fft1": 120.0, "fft2": 90.0}
-    f = scopes.parse_scopes(DEVICE)
-    assert scopes.validate_segments(seg, f, "long_device", "fused2") == []
-    assert scopes.validate_segments({"fft1": 210.0}, f, "long_device", "single") == []
-    # segments that do not add up to the chain, or belong to another chain
-    assert any("telescope" in p for p in
-               scopes.validate_segments({"fft1": 500.0, "fft2": 90.0}, f, "long_device", "fused2"))
-    assert any("missing" in p for p in scopes.validate_segments({"fft1": 210.0}, f, "long_device", "fused2"))
-    assert any("not part" in p for p in scopes.validate_segments(seg, f, "long_device", "single"))
-    assert scopes.validate_segments(None, scopes.parse_scopes(SHORT), "short") == []
-    assert scopes.validate_segments(seg, scopes.parse_scopes(SHORT), "short") != []
-    six = {k: 35.0 for k in scopes.SEGMENT_FIELDS}
-    assert scopes.validate_segments(six, f, "long_device", "separate") == []
-    with pytest.raises(ValueError):
-        scopes.parse_segments("segments: fft3=1.0 us")
-    for impl in ("single", "fused2", "fused", "separate"):
-        assert scopes.parse_boundary_impl(f"boundary_impl: {impl}") == impl
-    with pytest.raises(ValueError):
-        scopes.parse_boundary_impl("boundary_impl: bespoke")
+    f = report.Scopes.parse(STAGED)
+    assert report.segment_problems(seg, f, "fused2") == []
+    assert report.segment_problems({"fft1": 210.0}, f, "single") == []
+    assert any("add up to" in p for p in report.segment_problems({"fft1": 500.0, "fft2": 90.0}, f, "fused2"))
+    assert any("is not timed" in p for p in report.segment_problems({"fft1": 210.0}, f, "fused2"))
+    assert any("does not belong" in p for p in report.segment_problems(seg, f, "single"))
+    resident = report.Scopes.parse(RESIDENT)
+    assert report.segment_problems(None, resident, None) == []
+    assert report.segment_problems(seg, resident, None) != []
+    six = {k: 35.0 for k in report.LAUNCH_NAMES}
+    assert report.segment_problems(six, f, "separate") == []
+    with pytest.raises(report.ReportError):
+        report.parse_segments("segments: fft3=1.0 us")
+    for form in report.CHAINS:
+        assert report.parse_chain(f"boundary_impl: {form}") == form
+    with pytest.raises(report.ReportError):
+        report.parse_chain("boundary_impl: bespoke")
+
+
+def test_whole_report():
+    text = "\n".join([STAGED, "boundary_impl: fused2", "segments: fft1=120.0 us fft2=90.0 us",
+                      "n=65536 batch=2 launches=2  maxAbs=1e-3 maxRel=2.0e-07 (worst idx 5)",
+                      "bfly_c2c: 4100.0 us/call over 5 reps (min 3900.0 us) med 4100.0 us", "PASS"])
+    rep = report.Report.parse(text)
+    assert rep.problems() == [] and rep.verdict == "PASS" and rep.launches == 2 and rep.us_per_call == 4100.0
+    assert report.Report.parse(text.replace("PASS", "")).problems() == ["no PASS / FAIL line"]

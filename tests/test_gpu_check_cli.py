@@ -3,7 +3,7 @@ build/fft_check, src/host/fft_check.cpp:213-1315) on the MI355X.
 
 Every run must PASS its own gate (maxRelScaled <= 1e-4 per row against the
 library's FP64 reference), print timing lines that validate under
-bfly.scopes (short path: no transfers; long chains: h2d / d2h measured and
+bfly.report (resident runs: no transfers; staged long chains: h2d / d2h measured and
 segments adding up to device_chain), and its dumped output must meet the
 FP64 oracle at relative L2 <= 1e-5 per row.  AB_INPUT_SEQ must re-run one
 plan on changing inputs with no stale output.
@@ -16,7 +16,7 @@ import numpy as np
 import pytest
 
 import oracle
-from bfly import scopes
+from bfly import report
 from conftest import gpu_available
 
 pytestmark = [pytest.mark.gpu,
@@ -79,21 +79,17 @@ def test_cases_pass_and_report(tmp_path, n, batch, reps, envs):
     prefix = tmp_path / "d"
     r = run(n, batch, reps, AB_DUMP=prefix, **envs)
     assert r.returncode == 0, r.stdout + r.stderr
-    rep = scopes.parse_report(r.stdout)
-    assert rep["verdict"] == "PASS" and rep["max_rel"] <= 1e-4
-    f = rep["scopes"]
+    rep = report.Report.parse(r.stdout)
+    assert rep.verdict == "PASS" and rep.max_rel <= 1e-4
     kind = envs.get("AB_DIR", "c2c")
     is_long = kind == "c2c" and n > 4096
-    mode = scopes.infer_mode(f)
-    assert mode == ("long_device" if is_long else "short")
-    assert scopes.validate_scopes(f, mode) == []
+    assert rep.mode == ("staged" if is_long else "resident")
+    # one-decimal printing: allow 0.1 us per launch span beside 1 %
+    assert rep.problems(report.Tolerance(segments_us=0.25)) == []
     if is_long:
-        impl = rep["boundary_impl"]
-        assert impl == ("fused2" if rep["launches"] == 2 else "single")
-        # one-decimal printing: allow 0.1 us per segment beside 1 %
-        assert scopes.validate_segments(rep["segments"], f, mode, impl, abs_tol_us=0.25) == []
+        assert rep.chain == ("fused2" if rep.launches == 2 else "single")
     else:
-        assert rep["segments"] is None and rep["boundary_impl"] is None
+        assert rep.segments is None and rep.chain is None
     # the dumped output against the FP64 oracle
     x = np.fromfile(f"{prefix}.in.bin", np.float32)
     y = np.fromfile(f"{prefix}.out.bin", np.float32)
@@ -106,8 +102,8 @@ def test_cases_pass_and_report(tmp_path, n, batch, reps, envs):
 def test_input_sequence_reuses_the_plan_without_stale_outputs(n, batch):
     r = run(n, batch, 1, AB_INPUT_SEQ="sin,random-seeded,sin,impulse", AB_SEED=11)
     assert r.returncode == 0, r.stdout + r.stderr
-    rep = scopes.parse_report(r.stdout)
-    assert [(s[2], s[3]) for s in rep["seq"]] == [("PASS", False)] * 4
+    rep = report.Report.parse(r.stdout)
+    assert [(s.verdict, s.stale) for s in rep.seq] == [("PASS", False)] * 4
     assert "stale-check on -> PASS" in r.stdout
 
 
