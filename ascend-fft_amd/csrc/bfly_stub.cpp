@@ -22,6 +22,7 @@ struct Counters {
     int fail_allocation = 0;  // the Nth allocation fails (0 = never)
     int fail_copy = 0;        // the Nth copy fails (0 = never)
     int fail_launch = 0;      // the Nth launch fails (0 = never)
+    int last_c2c_variant = 0; // kernel of the last accepted C2C call (AUTO resolved)
 } g;
 int launch() {
     ++g.launches;
@@ -45,6 +46,9 @@ BFLY_API void bfly_stub_counters(int* out) {
                       g.streams_created, g.streams_destroyed, g.inits, g.syncs};
     std::memcpy(out, v, sizeof(v));
 }
+
+// BFLY_VARIANT_* the last accepted C2C execute call ran (0 before any)
+BFLY_API int bfly_stub_last_c2c_variant(void) { return g.last_c2c_variant; }
 
 // ------------------------------------------------------------- runtime
 int bfly_rt_init(int device) {
@@ -200,6 +204,7 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
     const bool uses_work = bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant);
     if (!bfly_host::disjoint(BFLY_KIND_C2C, in, out, uses_work ? work : nullptr, n, batch, 0, 0))
         return BFLY_ERR_SHAPE;
+    g.last_c2c_variant = variant;
     return launch();
 }
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,

@@ -60,6 +60,10 @@ struct Node {
         if (kind != Arr || items.empty()) throw std::runtime_error("expected a nonempty array");
         return items;
     }
+    const std::map<std::string, Node>& asObject() const {
+        if (kind != Obj || fields.empty()) throw std::runtime_error("expected a nonempty object");
+        return fields;
+    }
 };
 
 class Reader {

@@ -69,8 +69,9 @@ struct Generator {
 struct Mapping {
     int udCore = 256;      // compute units the grid may occupy
     int td = 1;            // tile reuse (1 = one transform tile per pass)
-    int tb = 1;            // batch loop granularity inside a workgroup
-    int ub = 1;            // transforms claimed per workgroup ("auto" resolves to 256/T)
+    int tb = 1;            // 1 = one tile of transforms per workgroup (one-shot grid);
+                           // 0 = a resident grid whose workgroups walk the batch
+    int ub = 1;            // tiles in flight per workgroup (1, or 2 in the two-deep streaming kernel)
     int ts = 1;            // 1 = all passes of a transform stay resident in LDS
     int us = 1;            // stage services per workgroup
     int level = 0;         // 0 = one GPU; 1 = multi-GPU (not implemented)

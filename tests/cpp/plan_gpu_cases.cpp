@@ -15,6 +15,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <memory>
+#include <sstream>
 #include <string>
 #include <vector>
 
@@ -171,6 +172,18 @@ int main(int argc, char** argv) {
         std::printf("case measure n %u batch %u rc %d us %.2f maxrel %.3e ok %d\n", n, b, rc, m.measuredUs,
                     m.maxRelErr, (int)m.measuredOk);
         if (rc != 0 || !m.measuredOk || !(m.measuredUs > 0.0)) g_fail++;
+    }
+
+    // the measured selection at the headline shape: every lowered kernel is
+    // measured (topK = 9) and its log printed line by line
+    {
+        std::string log;
+        auto plan = ctx.select(4096, 4096, 9, &log);
+        std::istringstream lines(log);
+        for (std::string l; std::getline(lines, l);) std::printf("selectlog %s\n", l.c_str());
+        if (!plan) g_fail++;
+        else std::printf("case select n 4096 batch 4096 kernel %s us %.2f\n", bfly::kernelName(plan->candidate().kernel),
+                         plan->candidate().q.measuredUs);
     }
 
     std::printf("plan gpu cases: %s (%d failures)\n", g_fail ? "FAILED" : "passed", g_fail);

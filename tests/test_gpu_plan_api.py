@@ -78,3 +78,23 @@ def test_plan_run_real_against_fp64_oracle(run, n):
         assert np.array_equal(h2, h)
         assert oracle.rel_l2_rows(y, oracle.c2r_f64(h2, n)) <= REL_L2
         assert np.max(np.abs(y - x)) < 1e-5
+
+
+def test_select_measures_every_kernel_and_keeps_the_fastest(run):
+    """Context::select at 4096 x 4096 measures each of the eight kernels the
+    space lowers to there (split MFMA included), each once, and returns the
+    fastest one that passed the gate."""
+    r, _ = run
+    log = [l[len("selectlog "):] for l in r.stdout.splitlines() if l.startswith("selectlog ")]
+    measured = {}
+    for l in log:
+        m = re.match(r"\s+(\S+)\s+eta=\s*(\S+) us\s+measured=\s*(\S+) us\s+maxRel=(\S+)", l)
+        if m:
+            assert m.group(1) not in measured, l
+            measured[m.group(1)] = float(m.group(3))
+            assert float(m.group(4)) <= 1e-4
+    assert set(measured) == {"plain", "stream", "plain_swz", "stream_swz", "stream_lt", "stream2",
+                             "split_mfma", "split_mfma_stream"}, log
+    assert len(set(measured.values())) >= 6                    # distinct measurements
+    sel = re.search(r"case select n 4096 batch 4096 kernel (\S+) us (\S+)", r.stdout)
+    assert sel and measured[sel.group(1)] == min(measured.values()), (sel and sel.group(0), measured)
