@@ -32,8 +32,8 @@ class C2C(torch.autograd.Function):
     the scale applied in the kernel's last store."""
     @staticmethod
     def forward(ctx, x: torch.Tensor, sign: int = -1, scale: float | None = None):
-        ctx.sign, ctx.scale = sign, scale
-        return ops.c2c(x.contiguous(), sign, scale=scale)
+        ctx.sign, ctx.scale = sign, 1.0 if scale is None else scale
+        return ops.c2c(x.contiguous(), sign, scale=ctx.scale)
 
     @staticmethod
     def backward(ctx, g: torch.Tensor):
@@ -107,13 +107,14 @@ def fft(x: torch.Tensor, norm: str | None = None) -> torch.Tensor:
     return C2C.apply(x, -1, ops.norm_scale(x.shape[-1], -1, norm))
 
 
-def ifft(x: torch.Tensor, norm: str | None = "backward") -> torch.Tensor:
-    """Differentiable inverse C2C; `norm` as in torch.fft.ifft ("backward":
-    1/n, "ortho": 1/sqrt(n), "forward": unscaled; None: unscaled)."""
+def ifft(x: torch.Tensor, norm: str | None = None) -> torch.Tensor:
+    """Differentiable inverse C2C; `norm` as in torch.fft.ifft (None and
+    "backward": 1/n, "ortho": 1/sqrt(n), "forward": unscaled)."""
     return C2C.apply(x, 1, ops.norm_scale(x.shape[-1], 1, norm))
 
 
 def ifft_unnormalized(x: torch.Tensor) -> torch.Tensor:
+    """The reference's unnormalised inverse (n times numpy's ifft)."""
     return C2C.apply(x, 1)
 
 

@@ -76,12 +76,11 @@ def c2c_raw(x: torch.Tensor, out: torch.Tensor, n: int, batch: int, sign: int = 
 
 
 def norm_scale(n: int, sign: int, norm: str | None) -> float | None:
-    """Output scale of numpy's `norm` conventions for a length-n C2C: None
-    (the reference's unnormalised transform both ways), "backward" (1/n on
-    the inverse), "ortho" (1/sqrt(n) both ways), "forward" (1/n on the
-    forward).  None means no scaling."""
-    if norm is None:
-        return None
+    """Output scale of a length-n C2C under numpy's / torch.fft's `norm`
+    conventions: "backward" (also None, their default) scales the inverse by
+    1/n, "ortho" both directions by 1/sqrt(n), "forward" the forward by 1/n.
+    Returns None where the transform is left unscaled."""
+    norm = "backward" if norm is None else norm
     if norm not in ("backward", "ortho", "forward"):
         raise ValueError(f"norm must be None, 'backward', 'ortho' or 'forward', not {norm!r}")
     if norm == "ortho":
@@ -91,9 +90,11 @@ def norm_scale(n: int, sign: int, norm: str | None) -> float | None:
 
 
 def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None, *, scale: float | None = None) -> torch.Tensor:
-    """C2C along the last axis (sign -1 forward, +1 inverse); unnormalised
-    unless `norm` names numpy's convention or `scale` gives the factor
-    outright, which the transform's last store applies."""
+    """C2C along the last axis (sign -1 forward, +1 inverse) with numpy's
+    `norm` convention (None = "backward": the forward unscaled, the inverse
+    scaled 1/n).  `scale` instead gives the factor outright, applied in the
+    transform's last store; scale=1.0 is the reference's unnormalised
+    transform in both directions."""
     if norm is not None and scale is not None:
         raise ValueError("give norm or scale, not both")
     _check_device(x, "x")
@@ -112,7 +113,8 @@ def c2c(x: torch.Tensor, sign: int = -1, norm: str | None = None, *, scale: floa
     if batch == 0:
         return out
     of = torch.view_as_real(out) if x.dtype == torch.complex64 else out
-    c2c_raw(xf, of, n, batch, sign, scale=scale if scale is not None else norm_scale(n, sign, norm))
+    factor = norm_scale(n, sign, norm) if scale is None else scale
+    c2c_raw(xf, of, n, batch, sign, scale=None if factor == 1.0 else factor)
     return out
 
 

@@ -236,7 +236,7 @@ def test_rows_and_scale_wrappers_refuse_bad_arguments_on_the_host():
         ops.c2c_rows(torch.zeros(2, 8, dtype=torch.complex64))
     with pytest.raises(ValueError):
         ops.c2c(torch.zeros(2, 8, dtype=torch.complex64), -1, norm="ortho", scale=0.5)
-    assert ops.norm_scale(64, -1, None) is None
+    assert ops.norm_scale(64, -1, None) is None and ops.norm_scale(64, 1, None) == 1.0 / 64
     assert ops.norm_scale(64, 1, "backward") == 1.0 / 64
     assert ops.norm_scale(64, -1, "forward") == 1.0 / 64
     assert ops.norm_scale(64, 1, "ortho") == ops.norm_scale(64, -1, "ortho") == 0.125
@@ -251,8 +251,7 @@ def test_norm_scale_follows_numpy(norm):
     for sign, fn in ((-1, np.fft.fft), (1, np.fft.ifft)):
         sc = ops.norm_scale(n, sign, norm)
         raw = np.fft.fft(x) if sign < 0 else np.fft.ifft(x) * n  # unnormalised both ways
-        want = fn(x, norm=norm) if norm is not None else raw
-        np.testing.assert_allclose(raw * (1.0 if sc is None else sc), want, rtol=1e-12, atol=1e-12)
+        np.testing.assert_allclose(raw * (1.0 if sc is None else sc), fn(x, norm=norm), rtol=1e-12, atol=1e-12)
     with pytest.raises(ValueError):
         ops.norm_scale(n, 1, "unitary")
 

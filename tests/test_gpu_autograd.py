@@ -35,7 +35,7 @@ def test_c2c_adjoint_matches_fp64_autograd(n):
     assert _rel(xd.grad.cpu().numpy(), x64.grad.numpy()) < 1e-6
 
 
-@pytest.mark.parametrize("norm", ["backward", "ortho", "forward"])
+@pytest.mark.parametrize("norm", [None, "backward", "ortho", "forward"])
 @pytest.mark.parametrize("n", [64, 4096, 16384])
 def test_normalised_c2c_gradients_match_torch_fft_fp64(n, norm):
     # fft / ifft with torch.fft's `norm` conventions: values and gradients
