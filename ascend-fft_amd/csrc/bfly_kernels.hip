@@ -482,7 +482,7 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float
 template <int I1, int N2, int DIR, int WG = bfly_plan::kBlock>
 __global__ __launch_bounds__(WG) void k_long_cols(
     const float2* __restrict__ in, float2* __restrict__ ws, const float2* __restrict__ tw1,
-    const float2* __restrict__ wN, uint64_t idist) {
+    const float2* __restrict__ wN, uint64_t idist, int temporal) {
     using PL = PlanOf<I1>;
     constexpr int N1 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -516,22 +516,33 @@ __global__ __launch_bounds__(WG) void k_long_cols(
     // the first LDS exchange of the FFT orders the table writes before any read
     run_plan_lt<I1, T, DIR, LX_PAD>(v, lds + c * STRIDE, 0, t1s, t1s, t1s, t, std::make_index_sequence<2>{});
     float2* dst = ws + b * N + n2;
+    // The intermediate is read back by the row kernel right after: from
+    // 2^24 points per launch up it is written (and read) with ordinary,
+    // temporal accesses (65536 x 256: 98.5 -> 93.3 us, 32768 x 512: 94.9 ->
+    // 87.1), below that with the streaming hint (65536 x 64: 28.2 against
+    // 30.6 us temporal).  `temporal` is uniform.
+    auto store = [&](auto tmp) {
 #pragma unroll
-    for (int m = 0; m < P / RL; m++)
+        for (int m = 0; m < P / RL; m++)
 #pragma unroll
-        for (int r = 0; r < RL; r++) {
-            const int k1 = t + m * T + r * (N1 / RL);
-            const int e = (n2 * k1) & (int)(N - 1);
-            const float2 a = tab[e & (NA - 1)], w = tab[NA + (e >> 8)];
-            const float2 wv = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
-            st_stream(&dst[(uint64_t)N2 * k1], cmul_tw<DIR>(v[m * RL + r], wv));
-        }
+            for (int r = 0; r < RL; r++) {
+                const int k1 = t + m * T + r * (N1 / RL);
+                const int e = (n2 * k1) & (int)(N - 1);
+                const float2 a = tab[e & (NA - 1)], w = tab[NA + (e >> 8)];
+                const float2 wv = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+                const float2 y = cmul_tw<DIR>(v[m * RL + r], wv);
+                if constexpr (decltype(tmp)::value) dst[(uint64_t)N2 * k1] = y;
+                else st_stream(&dst[(uint64_t)N2 * k1], y);
+            }
+    };
+    if (temporal) store(std::true_type{});
+    else store(std::false_type{});
 }
 
 template <int I2, int N1, int DIR, int WG = bfly_plan::kBlock>
 __global__ __launch_bounds__(WG) void k_long_rows(
     const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2, float scale,
-    uint64_t odist) {
+    uint64_t odist, int temporal) {
     using PL = PlanOf<I2>;
     constexpr int N2 = PL::N, T = PL::T, C = WG / T;
     constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
@@ -548,14 +559,20 @@ __global__ __launch_bounds__(WG) void k_long_rows(
     const uint64_t b = blockIdx.x / GROUPS;
     const int k1_0 = (blockIdx.x % GROUPS) * C;
     constexpr uint64_t N = (uint64_t)N1 * N2;
-    // Stage the C contiguous rows through LDS (row-linear, coalesced).
+    // Stage the C contiguous rows through LDS (row-linear, coalesced); the
+    // intermediate's access hint follows k_long_cols (`temporal` is uniform).
     const float2* src = ws + b * N + (uint64_t)k1_0 * N2;
+    auto stage = [&](auto tmp) {
 #pragma unroll
-    for (int i = 0; i < C * N2 / WG; i++) {
-        const int idx = tid + i * WG;
-        const int row = idx / N2, col = idx % N2;
-        lds[row * STRIDE + lds_idx(col)] = ld_stream(&src[idx]);
-    }
+        for (int i = 0; i < C * N2 / WG; i++) {
+            const int idx = tid + i * WG;
+            const int row = idx / N2, col = idx % N2;
+            if constexpr (decltype(tmp)::value) lds[row * STRIDE + lds_idx(col)] = src[idx];
+            else lds[row * STRIDE + lds_idx(col)] = ld_stream(&src[idx]);
+        }
+    };
+    if (temporal) stage(std::true_type{});
+    else stage(std::false_type{});
     __syncthreads();
     const int c = tid % C, t = tid / C;
     float2* L = lds + c * STRIDE;
@@ -765,19 +782,20 @@ hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const floa
     // measured slower in 512-thread workgroups.
     constexpr int C1w = 512 / PlanOf<I1>::T, C1 = bfly_plan::kBlock / PlanOf<I1>::T;
     constexpr int C2 = bfly_plan::kBlock / PlanOf<I2>::T;
+    const int temporal = (uint64_t)batch * N1 * N2 >= (1ull << 24) ? 1 : 0;
     static bfly_kern::ResidentCache rc;
     const int fit = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_long_cols<I1, N2, DIR, 512>), 512);
     if (fit > 0 && (uint64_t)batch * (N2 / C1w) >= (uint64_t)fit)
         hipLaunchKernelGGL((k_long_cols<I1, N2, DIR, 512>), dim3(batch * (N2 / C1w)), dim3(512), 0, s, in, ws,
-                           tw1, wN, idist);
+                           tw1, wN, idist, temporal);
     else
         hipLaunchKernelGGL((k_long_cols<I1, N2, DIR>), dim3(batch * (N2 / C1)), dim3(bfly_plan::kBlock), 0, s,
-                           in, ws, tw1, wN, idist);
+                           in, ws, tw1, wN, idist, temporal);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
     hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),
-                       0, s, ws, out, tw2, scale, odist);
+                       0, s, ws, out, tw2, scale, odist, temporal);
     return hipGetLastError();
 }
 
