@@ -89,10 +89,29 @@ def chosen_vs_plain(n: int, batch: int, pairs, host_input: np.ndarray, sample_ro
     plain = results[0]
     ok = [r for r in results if r["rel_l2"] <= REL_L2_GATE]
     best = min(ok, key=lambda r: r["ms"]) if ok else plain
+    cands = []
+    for r in results:
+        c = {"variant": VARIANT_NAMES.get(r["variant"], str(r["variant"])), "us": round(r["ms"] * 1e3, 2),
+             "rel_l2": r["rel_l2"]}
+        if r["variant"] in (_lib.VARIANT_SPLIT_MFMA, _lib.VARIANT_SPLIT_MFMA_STREAM):
+            flop = split_mfma_flop(n, batch)
+            c["mfma_tflops"] = round(flop / (r["ms"] * 1e-3) / 1e12, 1)
+            c["mfma_util"] = round(flop / (r["ms"] * 1e-3) / FP16_DENSE_PEAK, 4)
+        cands.append(c)
     return {
         "plan": VARIANT_NAMES.get(best["variant"], str(best["variant"])),
         "speedup_vs_plain": round(plain["ms"] / best["ms"], 4),
         "rel_l2_vs_fp64": best["rel_l2"],
-        "candidates": [{"variant": VARIANT_NAMES.get(r["variant"], str(r["variant"])),
-                        "us": round(r["ms"] * 1e3, 2), "rel_l2": r["rel_l2"]} for r in results],
+        "candidates": cands,
     }
+
+
+FP16_DENSE_PEAK = 2.5e15  # MI355X dense fp16 matrix flop/s (published, no sparsity)
+
+
+def split_mfma_flop(n: int, batch: int) -> float:
+    """Matrix flop of the split-FP16 path: per radix-16 pass, n/16 dense 16 x
+    16 complex products per transform (a 32 x 32 real block times a 32-vector:
+    2048 flop), three split products each."""
+    passes = int(np.log2(n)) // 4
+    return float(passes * (n // 16) * 2048 * 3 * batch)
