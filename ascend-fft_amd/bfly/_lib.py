@@ -24,6 +24,8 @@ ERR_LAUNCH = -5
 ERR_SYNC = -6
 ERR_D2H = -7
 ERR_SHAPE = -8
+ERR_SCRATCH = -9
+NOALLOC_SCRATCH_BYTES = 64 << 20
 
 VARIANT_AUTO = 0
 VARIANT_PLAIN = 1
@@ -53,6 +55,7 @@ ERRORS = {
     ERR_SYNC: "stream synchronisation failed",
     ERR_D2H: "device-to-host copy failed",
     ERR_SHAPE: "shape / pointer / size guard",
+    ERR_SCRATCH: "intermediate exceeds the static scratch of bfly_c2c_noalloc",
 }
 
 # Every symbol include/bfly_hip.h declares (tests check the export table).
@@ -68,6 +71,7 @@ SYMBOLS = (
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_fwd", "bfly_c2c_inv", "bfly_c2c_scaled", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
     "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
     "bfly_group_query", "bfly_group_destroy", "bfly_mode_mix", "bfly_probe_copy", "bfly_probe_device",
+    "bfly_noalloc_prepare", "bfly_c2c_noalloc",
 )
 
 
@@ -132,6 +136,8 @@ def _bind(lib):
         "bfly_c2c_fwd": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_inv": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_scaled": (i32, [vp, vp, u32, u32, i32, C.c_float, vp, vp, vp]),
+        "bfly_c2c_noalloc": (i32, [vp, vp, u32, u32, i32, vp, vp]),
+        "bfly_noalloc_prepare": (i32, []),
         "bfly_c2c_rows": (i32, [vp, C.c_uint64, vp, C.c_uint64, u32, u32, i32, vp, vp, vp]),
         "bfly_rt_teardown": (i32, []),
         "bfly_r2c": (i32, [vp, vp, u32, u32, vp, vp]),

@@ -26,7 +26,9 @@ int bfly_rt_init(int device) {
     int count = 0;
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return BFLY_ERR_NOT_LOADED;
     if (device < 0 || device >= count) return BFLY_ERR_TABLE;
-    return hipSetDevice(device) == hipSuccess ? BFLY_OK : BFLY_ERR_TABLE;
+    if (hipSetDevice(device) != hipSuccess) return BFLY_ERR_TABLE;
+    // the allocation-free entry point's scratch address, once per device
+    return bfly_noalloc_prepare() == BFLY_OK ? BFLY_OK : BFLY_ERR_TABLE;
 }
 
 int bfly_rt_teardown(void) { return hipDeviceSynchronize() == hipSuccess ? BFLY_OK : BFLY_ERR_SYNC; }

@@ -207,6 +207,19 @@ int bfly_c2c_ex(const float* in, float* out, uint32_t n, uint32_t batch, int sig
     g.last_c2c_variant = variant;
     return launch();
 }
+int bfly_noalloc_prepare(void) { return BFLY_OK; }
+int bfly_c2c_noalloc(const float* in, float* out, uint32_t n, uint32_t batch, int sign, const float* table,
+                     void* stream) {
+    static unsigned char scratch[BFLY_NOALLOC_SCRATCH_BYTES];  // host stand-in: the stub computes nothing
+    const bool known = bfly_supported(BFLY_KIND_C2C, n) && batch;
+    const int variant = known ? bfly_host::auto_variant(n, batch) : BFLY_VARIANT_AUTO;
+    const bool chain = known && bfly_host::needs_work(BFLY_KIND_C2C, n, batch, variant);
+    if (int g = bfly_host::check_execute(BFLY_KIND_C2C, sign, in, out, n, batch, table, chain ? scratch : nullptr,
+                                         variant))
+        return g;
+    if (chain && (uint64_t)n * batch * 8ull > sizeof(scratch)) return BFLY_ERR_SCRATCH;
+    return bfly_c2c_ex(in, out, n, batch, sign, variant, table, chain ? scratch : nullptr, stream);
+}
 int bfly_c2c(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
              const float* table, void* work, void* stream) {
     return bfly_c2c_ex(in, out, n, batch, sign, BFLY_VARIANT_AUTO, table, work, stream);

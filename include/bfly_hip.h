@@ -43,6 +43,8 @@ extern "C" {
 #define BFLY_ERR_SYNC (-6)        /* stream synchronisation failed             */
 #define BFLY_ERR_D2H (-7)         /* device-to-host copy failed                */
 #define BFLY_ERR_SHAPE (-8)       /* shape / pointer / size guard              */
+#define BFLY_ERR_SCRATCH (-9)     /* bfly_c2c_noalloc: the shape's intermediate
+                                     exceeds BFLY_NOALLOC_SCRATCH_BYTES       */
 
 #define BFLY_KIND_C2C 0
 #define BFLY_KIND_R2C 1
@@ -166,6 +168,24 @@ BFLY_API int bfly_c2c_inv(const float* in, float* out, uint32_t n, uint32_t batc
  * `work` as for bfly_c2c. */
 BFLY_API int bfly_c2c_scaled(const float* in, float* out, uint32_t n, uint32_t batch, int sign, float scale,
                              const float* table, void* work, void* stream);
+
+/* bfly_c2c for callers that cannot allocate device memory (code under HIP
+ * graph capture, tight serving loops, harnesses that forbid hipMalloc): same
+ * arguments minus `work`, same kernels, same results bit for bit.  No
+ * counterpart in the reference.  The four-step chain's intermediate (the
+ * only scratch a C2C uses: n > 16384, and 16384 with batch < 256) lives in a
+ * static device array of BFLY_NOALLOC_SCRATCH_BYTES per GPU, held until the
+ * process ends; a shape whose intermediate (8 n batch bytes) does not fit
+ * returns BFLY_ERR_SCRATCH (at 64 MiB: n * batch <= 8388608, e.g. 65536 x
+ * 128).  The call allocates nothing and frees nothing.  Every call on a GPU
+ * shares that GPU's array: calls that overlap on one GPU (other streams or
+ * host threads) race.  bfly_rt_init looks the array up for its device;
+ * bfly_noalloc_prepare does it for the current device (otherwise the first
+ * call that needs it does). */
+#define BFLY_NOALLOC_SCRATCH_BYTES (64u << 20)
+BFLY_API int bfly_noalloc_prepare(void);
+BFLY_API int bfly_c2c_noalloc(const float* in, float* out, uint32_t n, uint32_t batch, int sign,
+                              const float* table, void* stream);
 
 /* ------------------------------------------------------------------ 2-D
  * 2-D C2C of `batch` row-major [n1][n2] arrays of interleaved complex
