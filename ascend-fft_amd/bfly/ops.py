@@ -61,18 +61,20 @@ def c2c_raw(x: torch.Tensor, out: torch.Tensor, n: int, batch: int, sign: int = 
             work: torch.Tensor | None = None, scale: float | None = None) -> None:
     """bfly_c2c on interleaved float32 buffers (no allocation except tables);
     bfly_c2c_scaled when `scale` is given."""
+    lib = _lib.hip()
     tab = table(_lib.KIND_C2C, n, x.device)
-    if work is None and _lib.hip().bfly_workspace_bytes(_lib.KIND_C2C, n, batch):
+    if work is None and lib.bfly_workspace_bytes(_lib.KIND_C2C, n, batch):
         work = workspace(_lib.KIND_C2C, n, batch, x.device)
     wp = work.data_ptr() if work is not None else None
     if scale is None:
-        rc = _lib.hip().bfly_c2c(x.data_ptr(), out.data_ptr(), n, batch, sign, tab.data_ptr(), wp,
-                                 _stream(x.device))
-        _lib.check(rc, f"bfly_c2c(n={n}, batch={batch}, sign={sign})")
+        rc = lib.bfly_c2c(x.data_ptr(), out.data_ptr(), n, batch, sign, tab.data_ptr(), wp, _stream(x.device))
+        if rc:
+            _lib.check(rc, f"bfly_c2c(n={n}, batch={batch}, sign={sign})")
     else:
-        rc = _lib.hip().bfly_c2c_scaled(x.data_ptr(), out.data_ptr(), n, batch, sign, scale, tab.data_ptr(), wp,
-                                        _stream(x.device))
-        _lib.check(rc, f"bfly_c2c_scaled(n={n}, batch={batch}, sign={sign}, scale={scale})")
+        rc = lib.bfly_c2c_scaled(x.data_ptr(), out.data_ptr(), n, batch, sign, scale, tab.data_ptr(), wp,
+                                 _stream(x.device))
+        if rc:
+            _lib.check(rc, f"bfly_c2c_scaled(n={n}, batch={batch}, sign={sign}, scale={scale})")
 
 
 def norm_scale(n: int, sign: int, norm: str | None) -> float | None:

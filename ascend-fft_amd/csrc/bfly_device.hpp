@@ -402,6 +402,33 @@ __device__ __forceinline__ float2 tw2(const float2* tlo, const float2* thi, int 
     }
 }
 
+// The R-1 twiddles W^{r e} of one butterfly: W^e, W^{2e}, W^{4e}, W^{8e}
+// from the two-level tables, every other power as the product of its
+// highest power-of-two factor and the rest (w3 = w2 w1, ..., w15 = w8 w7:
+// at most three products deep).  A radix-16 butterfly then reads 8 LDS
+// values instead of 30 for the same complex products; the added rounding
+// is a few ulp per twiddle (transform error 1.27e-7 -> 1.40e-7 at 4096).
+template <int N, int R>
+__device__ __forceinline__ void tw_powers(float2 (&w)[R], const float2* tlo, const float2* thi, int e) {
+#pragma unroll
+    for (int r = 1; r < R; r *= 2) w[r] = tw2<N>(tlo, thi, r * e);
+#pragma unroll
+    for (int r = 3; r < R; r++) {
+        if ((r & (r - 1)) == 0) continue;
+        int hb = 1;
+        while (hb * 2 <= r) hb *= 2;
+        const float2 a = w[hb], b = w[r - hb];
+        w[r] = make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+    }
+}
+
+// Twiddles of the passes after the second built by tw_powers (fewer LDS reads
+// in the LDS-latency-bound one-workgroup 16384 kernel: 65.8 -> 61.9 us at
+// batch 1024; 47.0 -> 46.6 us at 4096 x 4096), except at 8192, where taking
+// each twiddle from the tables measured faster (57.5 vs 59.2 us at batch 2048).
+template <int N>
+constexpr bool kTwPowers = N != 8192;
+
 // `t1` holds the second pass's own table in LDS, transposed to [r Ns + k]
 // (Ns = the first radix) so that the lanes of a wave, whose k are
 // consecutive, read consecutive banks: one read per twiddle there; later
@@ -414,11 +441,17 @@ __device__ __forceinline__ void stockham_lt(float2 (&v)[P], float2* lds, int rb,
 #pragma unroll
         for (int m = 0; m < M; m++) {
             const int k = (t + m * T) & (Ns - 1);
+            if constexpr (Ns == bfly_plan::kPoints) {
 #pragma unroll
-            for (int r = 1; r < R; r++) {
-                if constexpr (Ns == bfly_plan::kPoints)
-                    v[m * R + r] = cmul_tw<DIR>(v[m * R + r], t1[r * Ns + k]);
-                else
+                for (int r = 1; r < R; r++) v[m * R + r] = cmul_tw<DIR>(v[m * R + r], t1[r * Ns + k]);
+            } else if constexpr (kTwPowers<N>) {
+                float2 w[R];
+                tw_powers<N, R>(w, tlo, thi, k * (N / (Ns * R)));
+#pragma unroll
+                for (int r = 1; r < R; r++) v[m * R + r] = cmul_tw<DIR>(v[m * R + r], w[r]);
+            } else {
+#pragma unroll
+                for (int r = 1; r < R; r++)
                     v[m * R + r] = cmul_tw<DIR>(v[m * R + r], tw2<N>(tlo, thi, r * k * (N / (Ns * R))));
             }
         }

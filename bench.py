@@ -150,6 +150,21 @@ def cpu_libraries(budget_s: float = 2.0):
     return out
 
 
+def clock_up(fn, seconds: float = 0.25, per_sync: int = 20):
+    """Untimed: run fn(i) back to back for `seconds` of wall time.  The GPU
+    leaves its idle clocks only after some milliseconds of sustained work, so
+    a workload timed right after host-only work (the CPU baselines) with a
+    few warm-up launches reads up to ~12 % slow (C2R n=4096 b=4096: 31.4 us
+    after a pause against 27.7 us clocked up)."""
+    import torch
+    t0, i = time.perf_counter(), 0
+    while time.perf_counter() - t0 < seconds:
+        for _ in range(per_sync):
+            fn(i)
+            i += 1
+        torch.cuda.synchronize()
+
+
 def copy_ceiling():
     """The device's measured streaming ceiling, as the hardware profile
     records it: bfly_probe_copy (the non-temporal four-deep float4 copy) at
@@ -166,8 +181,8 @@ def secondary_rows(cpu_seconds: float, reps: int = 20, rounds: int = 3):
     4096x4096 (with the reference's refR2CF32 / refC2RF32 timed on one host
     core over a bounded sample), long C2C at 65536 (b = 64 and 256) and
     16384 (b = 1024).  Kernel time by HIP events on the launch stream, the
-    reference's protocol (docs/benchmarks/methodology.md:52-55): warm-up,
-    then `rounds` rounds of `reps` launches, the minimum of the round means."""
+    reference's protocol (docs/benchmarks/methodology.md:52-55): warm-up
+    (0.25 s of back-to-back launches, clock_up), then `rounds` rounds of `reps` launches, the minimum of the round means."""
     import numpy as np
     import torch
     from bfly import _lib, ops
@@ -178,9 +193,7 @@ def secondary_rows(cpu_seconds: float, reps: int = 20, rounds: int = 3):
     sid = lambda: torch.cuda.current_stream().cuda_stream  # noqa: E731
 
     def timed(fn):
-        for i in range(5):
-            fn(i)
-        torch.cuda.synchronize()
+        clock_up(fn)
         best = None
         for _ in range(rounds):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -574,6 +587,7 @@ def workload_loop(name: str, steps: int, warmup: int):
         nbytes = 16 * n * b
     else:
         raise SystemExit(f"unknown workload {name}")
+    clock_up(lambda i: fn())
     for _ in range(warmup):
         fn()
     torch.cuda.synchronize()
@@ -610,10 +624,22 @@ def main():
         x = torch.from_numpy(host).to(dev) if i == 0 else torch.roll(pairs[0][0], shifts=i, dims=0)
         pairs.append((x.contiguous(), torch.empty_like(x)))
     stream = torch.cuda.current_stream(dev)
+    # One step = one bfly_c2c call through the C ABI, its arguments prepared
+    # once (as a C or FFI caller holds them): the host then spends a few us
+    # per launch, so the timed region is the kernels plus one launch latency
+    # and one synchronisation, not Python argument marshalling.
+    from bfly import _lib
+    lib = _lib.hip()
+    tab = ops.table(_lib.KIND_C2C, N, dev)
+    assert lib.bfly_workspace_bytes(_lib.KIND_C2C, N, BATCH) == 0
+    step_args = [(x.data_ptr(), y.data_ptr(), N, BATCH, -1, tab.data_ptr(), None, stream.cuda_stream)
+                 for x, y in pairs]
+    c2c = lib.bfly_c2c
 
     def step(i):
-        x, y = pairs[i % len(pairs)]
-        ops.c2c_raw(x, y, N, BATCH, -1)
+        rc = c2c(*step_args[i % len(step_args)])
+        if rc:
+            _lib.check(rc, f"bfly_c2c(n={N}, batch={BATCH})")
 
     # untimed: bring the GPU out of its idle clocks before the warmup steps
     # (a short warmup alone leaves the first timed steps a few % slow)
