@@ -291,10 +291,17 @@ __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], floa
     __syncthreads();
     if (tr < batch) {
         float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
+        // Rows of n+2 floats start 8 bytes further into a 128-byte line each
+        // time.  Where a wave's store covers only a few lines (T <= 64), the
+        // bins are shifted by the row's offset in its line, so that every
+        // store starts on a line boundary (same bins, same arithmetic, other
+        // lanes): R2C 1024 x 16384 31.5 -> 27.7 us, 2048 x 8192 28.2 -> 27.9;
+        // at T = 128 it measured neutral to slightly slower (4096 x 4096).
+        const int s = (T >= 16 && T <= 64) ? (int)((reinterpret_cast<uintptr_t>(dst) >> 3) & 15) : 0;
 #pragma unroll
         for (int i = 0; i <= P; i++) {
-            const int k = t + i * T;
-            if (k <= M && k < (int)kout) {
+            const int k = t + i * T - s;
+            if (k >= 0 && k <= M && k < (int)kout) {
                 const float2 a = L[lds_idx(k & (M - 1))];
                 const float2 b = L[lds_idx((M - k) & (M - 1))];
                 const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
