@@ -272,23 +272,49 @@ DEFAULT_SPACE = {
 class TuneResult:
     best: Config
     best_ms: float
-    table: list = field(default_factory=list)  # (config name, ms, max rel diff, units, grid)
+    table: list = field(default_factory=list)  # (config name, ms, max rel diff, units, grid, p50 us or None)
+    best_p50_us: float | None = None
+
+
+def pick(rows: list, gate: float, latency_slack: float | None = None) -> int:
+    """Index of the configuration to keep among `rows` of (ms, max rel diff,
+    p50 latency us or None).  Throughput objective (latency_slack None): the
+    fastest row within the accuracy gate.  Serving objective: among the
+    gated rows at most (1 + latency_slack) times slower than the fastest,
+    the one with the lowest median request latency (ties: the faster)."""
+    ok = [i for i, (ms, err, _) in enumerate(rows) if err <= gate]
+    if not ok:
+        raise RuntimeError("no grouped configuration met the accuracy gate")
+    fastest = min(ok, key=lambda i: rows[i][0])
+    if latency_slack is None:
+        return fastest
+    if latency_slack < 0:
+        raise ValueError("latency_slack must be >= 0")
+    limit = rows[fastest][0] * (1.0 + latency_slack)
+    near = [i for i in ok if rows[i][0] <= limit and rows[i][2] is not None]
+    if not near:
+        return fastest
+    return min(near, key=lambda i: (rows[i][2], rows[i][0]))
 
 
 def autotune(reqs: list[Request], space: dict | None = None, reps: int = 10,
-             gate: float = 1e-5, reference=None) -> TuneResult:
+             gate: float = 1e-5, reference=None, latency_slack: float | None = None) -> TuneResult:
     """Times every configuration of `space` on the same requests and keeps
     the fastest whose outputs stay within `gate` (relative L2 per request) of
     `reference` (default: per-request PLAIN launches).  Each candidate is
-    timed as a captured hipGraph replay, the form a serving loop runs."""
+    timed as a captured hipGraph replay, the form a serving loop runs.  With
+    `latency_slack` (e.g. 0.05) every gated candidate's per-request
+    completion times are also measured (one stamped launch) and the kept one
+    is the lowest median latency within that fraction of the fastest time
+    (`pick`): a serving loop trades a few percent of throughput for its short
+    requests finishing first."""
     import torch
     space = dict(DEFAULT_SPACE if space is None else space)
     if reference is None:
         launch_per_request(reqs)
         torch.cuda.synchronize()
         reference = outputs(reqs)
-    table = []
-    best, best_ms = None, float("inf")
+    table, cfgs = [], []
     keys = list(space)
     for vals in itertools.product(*(space[k] for k in keys)):
         cfg = Config(**dict(zip(keys, vals)))
@@ -303,11 +329,12 @@ def autotune(reqs: list[Request], space: dict | None = None, reps: int = 10,
         err = max_rel_diff(outputs(reqs), reference)
         graph = capture(grp.launch)
         ms = time_ms(graph.replay, reps=reps)
-        table.append((cfg.name(), ms, err, grp.units, grp.grid))
-        if err <= gate and ms < best_ms:
-            best, best_ms = cfg, ms
+        p50 = None
+        if latency_slack is not None and err <= gate:
+            p50 = float(np.percentile(latencies_us(grp), 50))
+        table.append((cfg.name(), ms, err, grp.units, grp.grid, p50))
+        cfgs.append(cfg)
         del graph
         grp.close()
-    if best is None:
-        raise RuntimeError("no grouped configuration met the accuracy gate")
-    return TuneResult(best, best_ms, table)
+    i = pick([(ms, err, p50) for _, ms, err, _, _, p50 in table], gate, latency_slack)
+    return TuneResult(cfgs[i], table[i][1], table, table[i][5])

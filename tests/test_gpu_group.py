@@ -246,9 +246,32 @@ def test_autotune_picks_a_gated_configuration():
     res = group.autotune(reqs, space={"workgroup": (512, 1024), "schedule": (0, 1), "order": (0, 1),
                                       "dynamic": (0, 1), "grid": (0,), "twiddles": (0, 1)}, reps=3)
     assert len(res.table) == 24  # LDS twiddles with schedule 0 only
-    assert all(err <= REL_L2 for _, _, err, _, _ in res.table)
+    assert all(err <= REL_L2 for _, _, err, _, _, _ in res.table)
     # schedule-0 candidates with device-memory twiddles are bit-exact with
     # the per-request PLAIN launches
-    assert all(err == 0.0 for name, _, err, _, _ in res.table if "_sched0_" in name and name.endswith("_tw0"))
-    assert res.best_ms == min(ms for _, ms, _, _, _ in res.table)
+    assert all(err == 0.0 for name, _, err, _, _, _ in res.table if "_sched0_" in name and name.endswith("_tw0"))
+    assert res.best_ms == min(ms for _, ms, _, _, _, _ in res.table)
+    torch.cuda.synchronize()
+
+
+def test_autotune_serving_objective_trades_time_for_latency():
+    """With latency_slack the kept configuration is within that fraction of
+    the fastest gated time and has the lowest median request latency there;
+    every gated row carries its measured median."""
+    import torch
+    from bfly import group
+    specs = group.uneven_specs(count=64, seed=5)
+    reqs = group.make_requests(specs, seed=5)
+    slack = 0.05
+    res = group.autotune(reqs, space={"workgroup": (512,), "schedule": (0,), "order": (0, 1, 3),
+                                      "dynamic": (1,), "grid": (0,), "twiddles": (1,)}, reps=5,
+                         latency_slack=slack)
+    assert len(res.table) == 3
+    gated = [row for row in res.table if row[2] <= REL_L2]
+    assert gated and all(row[5] is not None and row[5] > 0 for row in gated)
+    fastest = min(row[1] for row in gated)
+    assert res.best_ms <= fastest * (1 + slack)
+    near = [row for row in gated if row[1] <= fastest * (1 + slack)]
+    assert res.best_p50_us == min(row[5] for row in near)
+    assert res.best.name() in [row[0] for row in near]
     torch.cuda.synchronize()

@@ -154,3 +154,24 @@ def test_uneven_workload_generator():
     rc, h = _create(s, specs)
     assert rc == 0
     assert s.bfly_group_destroy(h) == 0
+
+
+def test_pick_throughput_and_serving_objectives():
+    # rows: (ms, max rel diff, p50 latency us)
+    rows = [(0.40, 1e-7, 300.0), (0.41, 1e-7, 25.0), (0.39, 2e-5, 10.0), (0.43, 1e-7, 20.0), (0.405, 0.0, None)]
+    # throughput: the fastest row inside the gate (row 2 is faster but fails it)
+    assert group.pick(rows, gate=1e-5) == 0
+    # serving: lowest median latency within 5 % of the fastest gated time
+    # (0.42 us limit: row 3 at 0.43 is out, row 4 has no latency)
+    assert group.pick(rows, gate=1e-5, latency_slack=0.05) == 1
+    assert group.pick(rows, gate=1e-5, latency_slack=0.10) == 3
+    assert group.pick(rows, gate=1e-5, latency_slack=0.0) == 0
+    # ties on latency go to the faster row
+    assert group.pick([(0.5, 0.0, 9.0), (0.45, 0.0, 9.0)], gate=1e-5, latency_slack=0.2) == 1
+    # no latency measured anywhere: the fastest
+    assert group.pick([(0.5, 0.0, None), (0.45, 0.0, None)], gate=1e-5, latency_slack=0.2) == 1
+    with pytest.raises(RuntimeError):
+        group.pick([(0.3, 1e-3, 5.0)], gate=1e-5)
+    with pytest.raises(ValueError):
+        group.pick(rows, gate=1e-5, latency_slack=-0.1)
+
