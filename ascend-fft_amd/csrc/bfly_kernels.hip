@@ -552,6 +552,88 @@ __global__ __launch_bounds__(WG) void k_long_cols(
     else store(std::false_type{});
 }
 
+// Streaming form of k_long_cols (512-thread workgroups): a resident grid
+// walks the units (C-column groups of every transform, in the same order as
+// the one-shot grid's workgroups) with the next unit's loads in flight
+// through the current unit's passes.  The loads go through a per-unit raw
+// buffer descriptor whose record count is zero past the last unit (no
+// branch around the prefetch; see k_c2c_single_pf), and the tables are
+// staged in LDS once per workgroup instead of once per unit.  Same
+// per-column arithmetic as k_long_cols, so the same bits.
+template <int I1, int N2, int DIR, bool TEMPORAL>
+__global__ __launch_bounds__(512) void k_long_cols_stream(
+    const float2* __restrict__ in, float2* __restrict__ ws, const float2* __restrict__ tw1,
+    const float2* __restrict__ wN, uint64_t idist, uint32_t units) {
+    using PL = PlanOf<I1>;
+    constexpr int WG = 512, N1 = PL::N, T = PL::T, C = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int STRIDE = lds_len(N1) + 1;
+    constexpr int GROUPS = N2 / C;
+    static_assert(N2 % C == 0 && PL::NRAD == 2, "two-pass column plans tiling N2");
+    constexpr int NA = 256, NB = (int)(((uint64_t)N1 * N2) / NA);
+    constexpr int R1 = kPlans[I1].rad[1];
+    constexpr uint64_t N = (uint64_t)N1 * N2;
+    // bytes from a unit's first element to one past its last
+    constexpr uint64_t kSpan = ((uint64_t)(N1 - 1) * N2 + C) * sizeof(float2);
+    static_assert(kSpan < (1ull << 31), "a unit's span fits a buffer descriptor");
+    __shared__ float2 lds[C * STRIDE];
+    __shared__ float2 tab[NA + NB];
+    __shared__ float2 t1s[bfly_plan::kPoints * R1];
+    const int tid = threadIdx.x;
+    for (int i = tid; i < NA + NB; i += WG) tab[i] = i < NA ? wN[i] : wN[(i - NA) * NA];
+    for (int i = tid; i < bfly_plan::kPoints * R1; i += WG)
+        t1s[(i % R1) * bfly_plan::kPoints + i / R1] = tw1[i];
+    // the first LDS exchange of the first unit orders the table writes before any read
+    const int c = tid % C, t0 = tid / C;
+    auto unit_rsrc = [&](uint32_t u, bool valid) {
+        const uint64_t b = u / GROUPS;
+        const int n2 = (int)(u % GROUPS) * C;
+        return __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(in + b * idist + n2), (short)0,
+                                                 valid ? (int)kSpan : 0, 0x00020000);
+    };
+    auto load_unit = [&](float2 (&v)[P], __amdgpu_buffer_rsrc_t rs, int t) {
+#pragma unroll
+        for (int m = 0; m < P / R0; m++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                const int row = t + m * T + r * (N1 / R0);
+                v[m * R0 + r] = __builtin_bit_cast(
+                    float2, __builtin_amdgcn_raw_buffer_load_b64(rs, (row * N2 + c) * (int)sizeof(float2), 0, 2));
+            }
+    };
+    float2 cur[P], nxt[P];
+    auto step = [&](uint32_t u) {
+        int t = t0;  // opaque per-step thread index (see k_r2c_stream)
+        asm volatile("" : "+v"(t));
+        const uint32_t un = u + gridDim.x;
+        load_unit(nxt, unit_rsrc(un < units ? un : u, un < units), t);
+        // the exchange's trailing barrier ends every LDS read before the
+        // next unit's first writes
+        run_plan_lt<I1, T, DIR, LX_PAD>(cur, lds + c * STRIDE, 0, t1s, t1s, t1s, t, std::make_index_sequence<2>{});
+        const uint64_t b = u / GROUPS;
+        const int n2 = (int)(u % GROUPS) * C + c;
+        float2* dst = ws + b * N + n2;
+#pragma unroll
+        for (int m = 0; m < P / RL; m++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                const int k1 = t + m * T + r * (N1 / RL);
+                const int e = (n2 * k1) & (int)(N - 1);
+                const float2 a = tab[e & (NA - 1)], w = tab[NA + (e >> 8)];
+                const float2 wv = make_float2(a.x * w.x - a.y * w.y, a.x * w.y + a.y * w.x);
+                const float2 y = cmul_tw<DIR>(cur[m * RL + r], wv);
+                if constexpr (TEMPORAL) dst[(uint64_t)N2 * k1] = y;
+                else st_stream(&dst[(uint64_t)N2 * k1], y);
+            }
+#pragma unroll
+        for (int i = 0; i < P; i++) cur[i] = nxt[i];
+    };
+    uint32_t u = blockIdx.x;  // the grid never exceeds the unit count
+    load_unit(cur, unit_rsrc(u, true), t0);
+    step(u);
+    for (u += gridDim.x; u < units; u += gridDim.x) step(u);
+}
+
 template <int I2, int N1, int DIR, int WG = bfly_plan::kBlock>
 __global__ __launch_bounds__(WG) void k_long_rows(
     const float2* __restrict__ ws, float2* __restrict__ out, const float2* __restrict__ tw2, float scale,
@@ -798,7 +880,33 @@ hipError_t launch_long_dir(const float2* in, float2* out, float2* ws, const floa
     const int temporal = (uint64_t)batch * N1 * N2 >= (1ull << 24) ? 1 : 0;
     static bfly_kern::ResidentCache rc;
     const int fit = bfly_kern::resident(rc, reinterpret_cast<const void*>(&k_long_cols<I1, N2, DIR, 512>), 512);
-    if (fit > 0 && (uint64_t)batch * (N2 / C1w) >= (uint64_t)fit)
+    const uint64_t units = (uint64_t)batch * (N2 / C1w);
+    // 256-point columns (n = 65536) with a column group for every resident
+    // slot: the streaming form on a resident grid (65536 x 256: 94.5 -> 88.3
+    // us, 65536 x 64: 30.4 -> 27.8, same bits); 128-point columns measured
+    // slower streamed (32768 x 128: 25.8 -> 27.8 us) and keep the one-shot grid.
+    if constexpr (N1 == 256) {
+        static bfly_kern::ResidentCache rcs[2];
+        const void* fs = temporal ? reinterpret_cast<const void*>(&k_long_cols_stream<I1, N2, DIR, true>)
+                                  : reinterpret_cast<const void*>(&k_long_cols_stream<I1, N2, DIR, false>);
+        const int fit_s = bfly_kern::resident(rcs[temporal ? 1 : 0], fs, 512);
+        if (fit_s > 0 && units >= (uint64_t)fit_s && units <= 0xffffffffull) {
+            const dim3 grid((unsigned)fit_s);
+            if (temporal)
+                hipLaunchKernelGGL((k_long_cols_stream<I1, N2, DIR, true>), grid, dim3(512), 0, s, in, ws, tw1, wN,
+                                   idist, (uint32_t)units);
+            else
+                hipLaunchKernelGGL((k_long_cols_stream<I1, N2, DIR, false>), grid, dim3(512), 0, s, in, ws, tw1, wN,
+                                   idist, (uint32_t)units);
+            hipError_t e = hipGetLastError();
+            if (e != hipSuccess) return e;
+            if (mid && (e = hipEventRecord(mid, s)) != hipSuccess) return e;
+            hipLaunchKernelGGL((k_long_rows<I2, N1, DIR>), dim3(batch * (N1 / C2)), dim3(bfly_plan::kBlock),
+                               0, s, ws, out, tw2, scale, odist, temporal);
+            return hipGetLastError();
+        }
+    }
+    if (fit > 0 && units >= (uint64_t)fit)
         hipLaunchKernelGGL((k_long_cols<I1, N2, DIR, 512>), dim3(batch * (N2 / C1w)), dim3(512), 0, s, in, ws,
                            tw1, wN, idist, temporal);
     else
