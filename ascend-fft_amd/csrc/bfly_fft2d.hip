@@ -105,6 +105,82 @@ __global__ __launch_bounds__(WG) void k_cols(const float2* __restrict__ in, floa
         }
 }
 
+// Streaming form of k_cols (no twiddle, every column group full): a
+// resident grid walks the units (C-column groups of every matrix, in the
+// one-shot grid's order) with the next unit's loads in flight through the
+// current unit's passes.  Loads and stores go through per-unit raw buffer
+// descriptors (zero records past the last unit, so no branch guards the
+// prefetch) with the per-row offset in the scalar operand, and the tables
+// are staged once per workgroup.  Same per-column arithmetic as k_cols, so
+// the same bits.  Used where one workgroup fills a CU (1024-point columns,
+// 139 KiB of LDS), which otherwise loads, computes and stores in turn.
+template <int I, int DIR, int WG>
+__global__ __launch_bounds__(WG) void k_cols_stream(const float2* __restrict__ in, float2* __restrict__ out,
+                                                    const float2* __restrict__ tw, ColView cv, uint32_t units,
+                                                    uint32_t in_span, uint32_t out_span) {
+    using PL = PlanOf<I>;
+    using LT = C2cLdsTw<I>;
+    constexpr int L = PL::N, T = PL::T, C = WG / T;
+    constexpr int R0 = PL::R0, RL = PL::RL, P = PL::P;
+    constexpr int STRIDE = lds_len(L) + 1;
+    typedef unsigned int u32x2 __attribute__((ext_vector_type(2)));
+    __shared__ float2 lds[C * STRIDE];
+    __shared__ float2 ltw[LT::LEN];
+    const int tid = threadIdx.x;
+    c2c_lds_tw<I, WG>(ltw, tw, tw + bfly_plan::stage_twiddle_len(I), tid);
+    // (the first unit's first LDS exchange orders these writes before the reads)
+    const int c = tid % C, t0 = tid / C;
+    // element offsets of unit u's first column, input and output side
+    auto base = [&](uint32_t u, uint64_t& oin, uint64_t& oout) {
+        const uint32_t g = u % cv.groups, mi = u / cv.groups;
+        const uint32_t b = mi / cv.mats, m = mi % cv.mats;
+        oin = (uint64_t)b * cv.item + (uint64_t)m * cv.in_mstride + (uint64_t)g * C;
+        oout = (uint64_t)b * cv.item + (uint64_t)m * cv.out_mstride + (uint64_t)g * C;
+    };
+    auto load_unit = [&](float2 (&v)[P], uint32_t u, bool valid, int t) {
+        uint64_t oin, oout;
+        base(u, oin, oout);
+        const auto rs = __builtin_amdgcn_make_buffer_rsrc(const_cast<float2*>(in + oin), (short)0,
+                                                          valid ? (int)in_span : 0, 0x00020000);
+        const uint32_t voff = ((uint32_t)t * cv.in_rstride + (uint32_t)c) * 8u;
+#pragma unroll
+        for (int mm = 0; mm < P / R0; mm++)
+#pragma unroll
+            for (int r = 0; r < R0; r++) {
+                const uint32_t soff = (uint32_t)(mm * T + r * (L / R0)) * cv.in_rstride * 8u;
+                v[mm * R0 + r] = __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(rs, voff, soff, 2));
+            }
+    };
+    float2 cur[P], nxt[P];
+    auto step = [&](uint32_t u) {
+        int t = t0;  // opaque per-step thread index (see k_r2c_stream)
+        asm volatile("" : "+v"(t));
+        const uint32_t un = u + gridDim.x;
+        load_unit(nxt, un < units ? un : u, un < units, t);
+        run_plan_lt<I, T, DIR, LX_PAD>(cur, lds + c * STRIDE, 0, ltw, ltw + LT::NLO, ltw + LT::NLO + LT::NHI, t,
+                                       std::make_index_sequence<PL::NRAD>{});
+        uint64_t oin, oout;
+        base(u, oin, oout);
+        const auto os = __builtin_amdgcn_make_buffer_rsrc(out + oout, (short)0, (int)out_span, 0x00020000);
+        const uint32_t voff = ((uint32_t)t * cv.out_rstride + (uint32_t)c) * 8u;
+#pragma unroll
+        for (int mm = 0; mm < P / RL; mm++)
+#pragma unroll
+            for (int r = 0; r < RL; r++) {
+                const uint32_t soff = (uint32_t)(mm * T + r * (L / RL)) * cv.out_rstride * 8u;
+                __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(u32x2, cur[mm * RL + r]), os, voff, soff, 2);
+            }
+        // the stores issue before the register copy waits for the prefetch
+        __builtin_amdgcn_sched_barrier(0);
+#pragma unroll
+        for (int i = 0; i < P; i++) cur[i] = nxt[i];
+    };
+    uint32_t u = blockIdx.x;  // the grid never exceeds the unit count
+    load_unit(cur, u, true, t0);
+    step(u);
+    for (u += gridDim.x; u < units; u += gridDim.x) step(u);
+}
+
 template <int I, bool TW, int WG = bfly_plan::kBlock>
 hipError_t launch_cols(const float2* in, float2* out, const float2* tw, const float2* wab, ColView cv,
                        uint64_t batch, int sign, hipStream_t s) {
@@ -112,6 +188,30 @@ hipError_t launch_cols(const float2* in, float2* out, const float2* tw, const fl
     cv.groups = (cv.ncols + C - 1) / C;
     const uint64_t blocks = batch * cv.mats * cv.groups;
     if (blocks == 0 || blocks > 0x7fffffffull) return hipErrorInvalidValue;
+    // 1024-point columns: the streaming form when every group is full, a
+    // unit's span fits a buffer descriptor and there is a unit for every
+    // resident slot (2-D 1024x1024 x16: 110.4 -> 106.4 us, 1024x128 x128:
+    // 95.6 -> 91.7; at 512 and 256 points, where two or more workgroups
+    // share a CU, it measured 0.5-2 % slower)
+    if constexpr (!TW && PlanOf<I>::N == 1024 && C >= 16) {
+        const uint64_t in_span = ((uint64_t)(PlanOf<I>::N - 1) * cv.in_rstride + C) * sizeof(float2);
+        const uint64_t out_span = ((uint64_t)(PlanOf<I>::N - 1) * cv.out_rstride + C) * sizeof(float2);
+        if (cv.ncols % C == 0 && in_span < (1ull << 31) && out_span < (1ull << 31)) {
+            static bfly_kern::ResidentCache rc[2];
+            const void* f = sign < 0 ? reinterpret_cast<const void*>(&k_cols_stream<I, -1, WG>)
+                                     : reinterpret_cast<const void*>(&k_cols_stream<I, 1, WG>);
+            const int fit = bfly_kern::resident(rc[sign < 0 ? 0 : 1], f, WG);
+            if (fit > 0 && blocks >= (uint64_t)fit) {
+                if (sign < 0)
+                    hipLaunchKernelGGL((k_cols_stream<I, -1, WG>), dim3((unsigned)fit), dim3(WG), 0, s, in, out, tw,
+                                       cv, (uint32_t)blocks, (uint32_t)in_span, (uint32_t)out_span);
+                else
+                    hipLaunchKernelGGL((k_cols_stream<I, 1, WG>), dim3((unsigned)fit), dim3(WG), 0, s, in, out, tw,
+                                       cv, (uint32_t)blocks, (uint32_t)in_span, (uint32_t)out_span);
+                return hipGetLastError();
+            }
+        }
+    }
     if (sign < 0)
         hipLaunchKernelGGL((k_cols<I, -1, TW, WG>), dim3((unsigned)blocks), dim3(WG), 0, s, in, out, tw, wab, cv);
     else

@@ -111,3 +111,41 @@ def test_strided_columns_match_the_oracle(dev, n, ncols, batch, sign):
     if sign < 0:
         ref = torch.fft.fft(x, dim=-2)
         assert (torch.linalg.vector_norm(y - ref) / torch.linalg.vector_norm(ref)).item() < 3e-6
+
+
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_streamed_column_pass_equals_the_one_shot_pass(dev, sign):
+    """1024-point columns with a full column group for every resident
+    workgroup run the streaming column kernel (the next group prefetched);
+    fewer groups run the one-shot kernel.  The per-column arithmetic is the
+    same, so strided columns transformed in one call (256 groups: streamed)
+    equal the same columns transformed in 512-wide slices (64 groups each:
+    one-shot) bit for bit; 2-D transforms whose column pass streams pass the
+    FP64 gate."""
+    import torch
+    from bfly import ops
+    g = torch.Generator(device="cpu").manual_seed(11 + sign)
+    for n1, n2, batch in ((1024, 1024, 5), (1024, 128, 40)):
+        print(f"fft2 streamed columns n1={n1} n2={n2} batch={batch} sign={sign}", flush=True)
+        x = torch.complex(torch.rand(batch, n1, n2, generator=g) * 2 - 1,
+                          torch.rand(batch, n1, n2, generator=g) * 2 - 1).to(dev)
+        y = ops.fft2(x, sign)
+        torch.cuda.synchronize()
+        z = torch.view_as_real(x).cpu().numpy().reshape(batch, n1, 2 * n2)
+        want = oracle.fft2_f64(z, n1, n2, sign)
+        got = np.ascontiguousarray(torch.view_as_real(y).cpu().numpy()).reshape(batch, -1)
+        assert oracle.rel_l2_rows(got, want) <= REL_L2
+    n, ncols, batch = 1024, 2048, 2
+    print(f"columns streamed n={n} ncols={ncols} batch={batch} sign={sign}", flush=True)
+    x = torch.complex(torch.rand(batch, n, ncols, generator=g) * 2 - 1,
+                      torch.rand(batch, n, ncols, generator=g) * 2 - 1).to(dev)
+    y = ops.fft_columns(x, sign)
+    # column blocks of 512 run with 32 groups each: the one-shot kernel
+    parts = torch.cat([ops.fft_columns(x[..., j:j + 512].contiguous(), sign) for j in range(0, ncols, 512)], dim=-1)
+    torch.cuda.synchronize()
+    assert torch.equal(y, parts)
+    xc = x.cpu().to(torch.complex128)
+    ref = torch.fft.fft(xc, dim=-2) if sign < 0 else torch.fft.ifft(xc, dim=-2) * n  # unnormalised inverse
+    rel = (torch.linalg.vector_norm(y.cpu().to(torch.complex128) - ref) / torch.linalg.vector_norm(ref)).item()
+    assert rel <= REL_L2
+
