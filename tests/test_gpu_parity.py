@@ -366,6 +366,24 @@ def test_single_workgroup_and_four_step_paths_agree(n, b):
                                   tab.data_ptr(), None, 0) == _lib.ERR_SHAPE
 
 
+@pytest.mark.parametrize("sign", [-1, 1])
+def test_streamed_long_column_pass_equals_the_one_shot_pass(sign):
+    # At n = 65536 the four-step column pass streams over its column groups
+    # once there is a group for every resident workgroup (batch >= 64 here);
+    # smaller batches run the one-shot column kernel.  The per-column
+    # arithmetic is the same: the batch of 64 in one call equals eight calls
+    # of 8 transforms bit for bit, and passes the FP64 gates.
+    import torch
+    n, b = 65536, 64
+    x = oracle.uniform_input(7, (b, 2 * n))
+    got = run_c2c(x, n, sign)
+    parts = np.concatenate([run_c2c(x[i:i + 8], n, sign) for i in range(0, b, 8)])
+    assert np.array_equal(got, parts)
+    rows = list(range(0, b, 9))
+    gate_c2c(got[rows], x[rows], n, sign)
+    torch.cuda.synchronize()
+
+
 def test_long_c2c_matches_host_four_step_split():
     # The reference's host four-step chain (src/host/fft_check.cpp:904-963) in
     # FP64 with its own split (fft_check.cpp:295-308) agrees with the direct
