@@ -71,7 +71,7 @@ SYMBOLS = (
     "bfly_workspace_bytes", "bfly_c2c", "bfly_c2c_fwd", "bfly_c2c_inv", "bfly_c2c_scaled", "bfly_c2c_ex", "bfly_r2c", "bfly_c2r", "bfly_r2c_ex",
     "bfly_c2r_ex", "bfly_real_ex", "bfly_group_supported", "bfly_group_create", "bfly_group_launch",
     "bfly_group_query", "bfly_group_destroy", "bfly_mode_mix", "bfly_probe_copy", "bfly_probe_device",
-    "bfly_noalloc_prepare", "bfly_c2c_noalloc",
+    "bfly_noalloc_prepare", "bfly_c2c_noalloc", "bfly_host_pipelined",
 )
 
 
@@ -137,6 +137,7 @@ def _bind(lib):
         "bfly_c2c_inv": (i32, [vp, vp, u32, u32, vp, vp, vp]),
         "bfly_c2c_scaled": (i32, [vp, vp, u32, u32, i32, C.c_float, vp, vp, vp]),
         "bfly_c2c_noalloc": (i32, [vp, vp, u32, u32, i32, vp, vp]),
+        "bfly_host_pipelined": (i32, [i32, i32, i32, vp, vp, u32, u32, vp, vp, vp, vp, u32, vp]),
         "bfly_noalloc_prepare": (i32, []),
         "bfly_c2c_rows": (i32, [vp, C.c_uint64, vp, C.c_uint64, u32, u32, i32, vp, vp, vp]),
         "bfly_rt_teardown": (i32, []),

@@ -6,6 +6,7 @@
 #include <cstdint>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 
 #include "bfly_hip.h"
 #include "bfly_kernels.hpp"
@@ -18,6 +19,65 @@ inline hipStream_t S(void* stream) { return reinterpret_cast<hipStream_t>(stream
 
 int launched(hipError_t e) { return e == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH; }
 
+// The transform and download streams of bfly_host_pipelined, and a ring of
+// events ordering each chunk's steps, created once per device (default
+// priority) and kept until the process ends.  An event is recorded again
+// only after the wait that used its previous recording was issued.
+constexpr int kPipeRing = 32;
+constexpr int kPipeDevices = 64;
+constexpr size_t kPipeChunkBytes = 8u << 20;
+struct PipeStreams {
+    hipStream_t comp = nullptr, down = nullptr;
+    hipEvent_t ev[2 * kPipeRing] = {};
+    bool ready = false;
+};
+PipeStreams g_pipe[kPipeDevices];
+std::mutex g_pipe_lock[kPipeDevices];
+
+// Creates the device's streams and events on first use (lock held).
+int pipe_ready(PipeStreams& P) {
+    if (P.ready) return BFLY_OK;
+    if (!P.comp && hipStreamCreateWithFlags(&P.comp, hipStreamNonBlocking) != hipSuccess) {
+        P.comp = nullptr;
+        (void)hipGetLastError();
+        return BFLY_ERR_ALLOC;
+    }
+    if (!P.down && hipStreamCreateWithFlags(&P.down, hipStreamNonBlocking) != hipSuccess) {
+        P.down = nullptr;
+        (void)hipGetLastError();
+        return BFLY_ERR_ALLOC;
+    }
+    for (hipEvent_t& e : P.ev)
+        if (!e && hipEventCreateWithFlags(&e, hipEventDisableTiming) != hipSuccess) {
+            e = nullptr;
+            (void)hipGetLastError();
+            return BFLY_ERR_ALLOC;
+        }
+    P.ready = true;
+    // a first kernel on the transform stream, a first recording of every
+    // event and a first download on the download stream, so that their
+    // set-up is paid here rather than in the first call (best effort: a
+    // failure leaves that set-up to the first call)
+    // (the download is large enough to go through the copy engine: small
+    // copies run as kernels)
+    constexpr size_t kWarm = 4u << 20;
+    void *d = nullptr, *h = nullptr;
+    if (hipMalloc(&d, kWarm) == hipSuccess && hipHostMalloc(&h, kWarm, hipHostMallocDefault) == hipSuccess) {
+        (void)hipMemsetAsync(d, 0, kWarm, P.comp);
+        for (int i = 0; i < 2 * kPipeRing; i++) {
+            (void)hipEventRecord(P.ev[i], i & 1 ? P.down : P.comp);
+            (void)hipStreamWaitEvent(i & 1 ? P.comp : P.down, P.ev[i], 0);
+        }
+        (void)hipMemcpyAsync(h, d, kWarm, hipMemcpyDeviceToHost, P.down);
+        (void)hipStreamSynchronize(P.down);
+        (void)hipStreamSynchronize(P.comp);
+    }
+    if (h) (void)hipHostFree(h);
+    if (d) (void)hipFree(d);
+    (void)hipGetLastError();
+    return BFLY_OK;
+}
+
 }  // namespace
 
 extern "C" {
@@ -27,6 +87,12 @@ int bfly_rt_init(int device) {
     if (hipGetDeviceCount(&count) != hipSuccess || count <= 0) return BFLY_ERR_NOT_LOADED;
     if (device < 0 || device >= count) return BFLY_ERR_TABLE;
     if (hipSetDevice(device) != hipSuccess) return BFLY_ERR_TABLE;
+    // the pipelined host call's streams and events, so that its first call
+    // does not create them (a failure here is retried by that call)
+    if (device < kPipeDevices) {
+        std::lock_guard<std::mutex> lock(g_pipe_lock[device]);
+        (void)pipe_ready(g_pipe[device]);
+    }
     // the allocation-free entry point's scratch address, once per device
     return bfly_noalloc_prepare() == BFLY_OK ? BFLY_OK : BFLY_ERR_TABLE;
 }
@@ -311,6 +377,81 @@ int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_
 int bfly_c2r(const float* in, float* out, uint32_t n, uint32_t batch, const float* table,
              void* stream) {
     return bfly_c2r_ex(in, out, n, batch, n / 2 + 1, n + 2, table, stream);
+}
+
+int bfly_host_pipelined(int kind, int sign, int variant, const float* host_in, float* host_out, uint32_t n,
+                        uint32_t batch, const float* table, void* dev_in, void* dev_out, void* work,
+                        uint32_t chunk_rows, void* stream) {
+    bfly_host::PipeShape sh;
+    if (int g = bfly_host::check_pipelined(kind, sign, variant, host_in, host_out, n, batch, table, dev_in, dev_out,
+                                           work, chunk_rows, &sh))
+        return g;
+    sign = sh.sign;
+    variant = sh.variant;
+    const size_t row_in = sh.row_in, row_out = sh.row_out;
+    const uint64_t cr = sh.chunk;
+    const hipStream_t up = S(stream);
+    if (cr >= batch) {
+        // one chunk: nothing to overlap, the three steps run on the caller's stream
+        if (hipMemcpyAsync(dev_in, host_in, batch * row_in * sizeof(float), hipMemcpyHostToDevice, up) != hipSuccess)
+            return BFLY_ERR_H2D;
+        const float* din = static_cast<const float*>(dev_in);
+        float* dout = static_cast<float*>(dev_out);
+        const hipError_t e =
+            kind == BFLY_KIND_C2C
+                ? bfly_kern::c2c(din, dout, n, batch, sign, variant, table, work, up, nullptr, 1.0f)
+                : bfly_kern::real(kind, din, dout, n, batch, n / 2 + 1, n + 2, variant, false, table, up);
+        int rc = e == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH;
+        if (rc == BFLY_OK &&
+            hipMemcpyAsync(host_out, dout, batch * row_out * sizeof(float), hipMemcpyDeviceToHost, up) != hipSuccess)
+            rc = BFLY_ERR_D2H;
+        const bool synced = hipStreamSynchronize(up) == hipSuccess;
+        if (rc != BFLY_OK) (void)hipGetLastError();
+        return rc == BFLY_OK && !synced ? BFLY_ERR_SYNC : rc;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= kPipeDevices) return BFLY_ERR_NOT_LOADED;
+    std::lock_guard<std::mutex> lock(g_pipe_lock[dev]);
+    PipeStreams& P = g_pipe[dev];
+    if (int e = pipe_ready(P)) return e;
+    int rc = BFLY_OK;
+    uint64_t k = 0;
+    for (uint64_t r0 = 0; r0 < batch; r0 += cr, k++) {
+        const uint32_t rows = (uint32_t)(batch - r0 < cr ? batch - r0 : cr);
+        const float* hin = host_in + r0 * row_in;
+        float* hout = host_out + r0 * row_out;
+        const float* din = static_cast<const float*>(dev_in) + r0 * row_in;
+        float* dout = static_cast<float*>(dev_out) + r0 * row_out;
+        hipEvent_t loaded = P.ev[2 * (k % kPipeRing)], done = P.ev[2 * (k % kPipeRing) + 1];
+        if (hipMemcpyAsync(const_cast<float*>(din), hin, rows * row_in * sizeof(float), hipMemcpyHostToDevice, up) !=
+            hipSuccess) {
+            rc = BFLY_ERR_H2D;
+            break;
+        }
+        if (hipEventRecord(loaded, up) != hipSuccess || hipStreamWaitEvent(P.comp, loaded, 0) != hipSuccess) {
+            rc = BFLY_ERR_LAUNCH;
+            break;
+        }
+        const hipError_t e =
+            kind == BFLY_KIND_C2C
+                ? bfly_kern::c2c(din, dout, n, rows, sign, variant, table, work, P.comp, nullptr, 1.0f)
+                : bfly_kern::real(kind, din, dout, n, rows, n / 2 + 1, n + 2, variant, false, table, P.comp);
+        if (e != hipSuccess || hipEventRecord(done, P.comp) != hipSuccess ||
+            hipStreamWaitEvent(P.down, done, 0) != hipSuccess) {
+            rc = BFLY_ERR_LAUNCH;
+            break;
+        }
+        if (hipMemcpyAsync(hout, dout, rows * row_out * sizeof(float), hipMemcpyDeviceToHost, P.down) != hipSuccess) {
+            rc = BFLY_ERR_D2H;
+            break;
+        }
+    }
+    // every issued step ends before the call returns, also after an error
+    const bool synced = hipStreamSynchronize(up) == hipSuccess && hipStreamSynchronize(P.comp) == hipSuccess &&
+                        hipStreamSynchronize(P.down) == hipSuccess;
+    if (rc != BFLY_OK) (void)hipGetLastError();
+    if (rc == BFLY_OK && !synced) rc = BFLY_ERR_SYNC;
+    return rc;
 }
 
 int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,

@@ -266,6 +266,31 @@ int bfly_c2r_ex(const float* in, float* out, uint32_t n, uint32_t batch, uint32_
                 uint32_t stride, const float* table, void* stream) {
     return bfly_real_ex(BFLY_KIND_C2R, in, out, n, batch, bins, stride, BFLY_VARIANT_AUTO, 0, table, stream);
 }
+// Same guards and chunking as the HIP library; the steps run one after the
+// other (copy in, launch, copy out per chunk), each counted and failable.
+int bfly_host_pipelined(int kind, int sign, int variant, const float* host_in, float* host_out, uint32_t n,
+                        uint32_t batch, const float* table, void* dev_in, void* dev_out, void* work,
+                        uint32_t chunk_rows, void* stream) {
+    bfly_host::PipeShape sh;
+    if (int e = bfly_host::check_pipelined(kind, sign, variant, host_in, host_out, n, batch, table, dev_in, dev_out,
+                                           work, chunk_rows, &sh))
+        return e;
+    for (uint64_t r0 = 0; r0 < batch; r0 += sh.chunk) {
+        const uint32_t rows = (uint32_t)(batch - r0 < sh.chunk ? batch - r0 : sh.chunk);
+        float* din = static_cast<float*>(dev_in) + r0 * sh.row_in;
+        float* dout = static_cast<float*>(dev_out) + r0 * sh.row_out;
+        if (int e = bfly_rt_memcpy_h2d(din, host_in + r0 * sh.row_in, rows * sh.row_in * sizeof(float), stream))
+            return e;
+        const int e = kind == BFLY_KIND_C2C
+                          ? bfly_c2c_ex(din, dout, n, rows, sh.sign, sh.variant, table, work, stream)
+                          : bfly_real_ex(kind, din, dout, n, rows, n / 2 + 1, n + 2, sh.variant, 0, table, stream);
+        if (e) return e;
+        if (int e2 = bfly_rt_memcpy_d2h(host_out + r0 * sh.row_out, dout, rows * sh.row_out * sizeof(float), stream))
+            return e2;
+    }
+    return BFLY_OK;
+}
+
 int bfly_rt_time_transform(int kind, int sign, const float* in, float* out, uint32_t n,
                            uint32_t batch, const float* table, void* work, int reps, void* stream,
                            float* ms) {

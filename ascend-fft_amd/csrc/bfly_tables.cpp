@@ -298,6 +298,48 @@ int auto_real_variant(uint32_t n, uint32_t batch) {
     return n >= 1024 ? BFLY_VARIANT_STREAM_LT : BFLY_VARIANT_STREAM;
 }
 
+int check_pipelined(int kind, int sign, int variant, const void* host_in, const void* host_out, uint32_t n,
+                    uint32_t batch, const void* table, const void* dev_in, const void* dev_out, const void* work,
+                    uint32_t chunk_rows, PipeShape* shape) {
+    if (!shape) return BFLY_ERR_SHAPE;
+    if (kind != BFLY_KIND_C2C && kind != BFLY_KIND_R2C && kind != BFLY_KIND_C2R) return BFLY_ERR_SHAPE;
+    if (kind != BFLY_KIND_C2C) sign = kind == BFLY_KIND_R2C ? -1 : 1;
+    if (!host_in || !host_out || !dev_in || !dev_out) return BFLY_ERR_SHAPE;
+    if (!shape_ok(kind, n)) return BFLY_ERR_SHAPE;
+    if (variant < BFLY_VARIANT_AUTO || variant > BFLY_VARIANT_LAST) return BFLY_ERR_SHAPE;
+    if (batch == 0) return BFLY_ERR_SHAPE;
+    const bool c2c = kind == BFLY_KIND_C2C;
+    shape->row_in = c2c ? 2 * (size_t)n : kind == BFLY_KIND_R2C ? (size_t)n : (size_t)n + 2;
+    shape->row_out = c2c ? 2 * (size_t)n : kind == BFLY_KIND_R2C ? (size_t)n + 2 : (size_t)n;
+    // about 8 MiB of input per chunk unless the caller says otherwise
+    uint64_t cr = chunk_rows ? chunk_rows : (8ull << 20) / (shape->row_in * sizeof(float));
+    if (cr < 1) cr = 1;
+    if (cr > batch) cr = batch;
+    shape->chunk = cr;
+    // the kernels run on chunks, so AUTO picks the variant for a chunk's batch
+    if (c2c) {
+        if (variant == BFLY_VARIANT_AUTO) variant = auto_variant(n, (uint32_t)cr);
+    } else {
+        if (variant == BFLY_VARIANT_AUTO) variant = auto_real_variant(n, (uint32_t)cr);
+        if (variant != BFLY_VARIANT_PLAIN && variant != BFLY_VARIANT_STREAM && variant != BFLY_VARIANT_STREAM_LT)
+            return BFLY_ERR_SHAPE;
+    }
+    if (int g = check_execute(kind, sign, dev_in, dev_out, n, batch, table, c2c ? work : nullptr, variant))
+        return g;
+    if (!disjoint(kind, dev_in, dev_out, nullptr, n, batch, n / 2 + 1, n + 2)) return BFLY_ERR_SHAPE;
+    // the workspace holds one chunk and lies apart from both whole-batch buffers
+    if (c2c && needs_work(BFLY_KIND_C2C, n, batch, variant)) {
+        const uintptr_t w = (uintptr_t)work, wb = 8ull * n * cr, ab = 8ull * n * batch;
+        for (const void* q : {dev_in, dev_out}) {
+            const uintptr_t a = (uintptr_t)q;
+            if (!(w + wb <= a || a + ab <= w)) return BFLY_ERR_SHAPE;
+        }
+    }
+    shape->sign = sign;
+    shape->variant = variant;
+    return BFLY_OK;
+}
+
 bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride) {
     // 1 <= bins <= n/2+1, the row holds the bins, rows stay 8-byte aligned
     return bins >= 1 && bins <= n / 2 + 1 && stride >= 2 * bins && (stride % 2) == 0;

@@ -81,6 +81,18 @@ bool bins_ok(uint32_t n, uint32_t bins, uint32_t stride);
 int auto_variant(uint32_t n, uint32_t batch);
 // The variant BFLY_VARIANT_AUTO resolves to for a real transform of (n, batch).
 int auto_real_variant(uint32_t n, uint32_t batch);
+// Guards and shape of one bfly_host_pipelined call (shared by the HIP library
+// and the CPU stub): resolves the direction of the real kinds and the AUTO
+// variant for one chunk's batch, checks the whole batch as one execute call
+// on the device buffers, and gives the row sizes (floats) and transforms per chunk.
+struct PipeShape {
+    int sign = -1, variant = 0;
+    size_t row_in = 0, row_out = 0;
+    uint64_t chunk = 0;
+};
+int check_pipelined(int kind, int sign, int variant, const void* host_in, const void* host_out, uint32_t n,
+                    uint32_t batch, const void* table, const void* dev_in, const void* dev_out, const void* work,
+                    uint32_t chunk_rows, PipeShape* shape);
 // The variant bfly_c2c_scaled runs: one whose last store applies the scale
 // (the LDS-twiddle streaming kernel up to n = 8192, AUTO's choice above).
 int scaled_variant(uint32_t n, uint32_t batch);

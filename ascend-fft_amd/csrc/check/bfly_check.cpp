@@ -19,7 +19,9 @@
 //                           and flag an output that did not change when its
 //                           input did (stale buffer)
 //   AB_E2E=<reps>           end-to-end loop: H2D, transform, D2H per rep;
-//                           AB_E2E_MODE=async|sync|xfer, AB_E2E_HOST=pinned|pageable
+//                           AB_E2E_MODE=async|sync|xfer|pipe, AB_E2E_HOST=pinned|pageable;
+//                           pipe = bfly_host_pipelined in chunks of AB_E2E_CHUNK
+//                           transforms (0: about 8 MiB of input), output checked
 //   AB_DUMP=<prefix>        write <prefix>.in.bin and <prefix>.out.bin
 //   AB_DEVICE_ID=<id>       device index
 //   AB_BOUNDARY=device      the long chain always runs on the device here;
@@ -412,12 +414,21 @@ int main(int argc, char** argv) {
     const double first_us = us_since(t_first);
 
     // end-to-end loop (off by default)
+    bool e2e_ok = true;
     if (const char* ee = env("AB_E2E")) {
         int ereps = std::atoi(ee);
         if (ereps <= 0) ereps = reps;
         const char* emode = env("AB_E2E_MODE");
         const bool use_async = !(emode && !std::strcmp(emode, "sync"));
         const bool xfer_only = emode && !std::strcmp(emode, "xfer");
+        const bool pipe = emode && !std::strcmp(emode, "pipe");
+        const uint32_t chunk = env("AB_E2E_CHUNK") ? (uint32_t)std::strtoul(env("AB_E2E_CHUNK"), nullptr, 10) : 0;
+        // a long C2C under AUTO may run a chunk as the four-step chain
+        if (pipe && kind == BFLY_KIND_C2C && !d_work && n > 4096 &&
+            bfly_rt_malloc(&d_work, 2 * (size_t)n * batch * sizeof(float)) != BFLY_OK) {
+            std::printf("E2E workspace allocation failed\n");
+            return 1;
+        }
         const char* hmode = env("AB_E2E_HOST");
         const bool want_pinned = !(hmode && !std::strcmp(hmode, "pageable"));
         float *p_in = nullptr, *p_out = nullptr;
@@ -442,7 +453,14 @@ int main(int argc, char** argv) {
         const auto t0e = Clock::now();
         for (int i = 0; i < ereps; i++) {
             const auto a0 = Clock::now();
-            if (is_long && !xfer_only) {
+            if (pipe) {
+                const int e = bfly_host_pipelined(kind, sign, variant, src, dst, n, batch, dtab, d_in, d_out, d_work,
+                                                  chunk, stream);
+                if (e) {
+                    std::printf("E2E pipelined=%d\n", e);
+                    return 1;
+                }
+            } else if (is_long && !xfer_only) {
                 if (!launch()) return 1;
             } else {
                 // async: queued copies with an explicit synchronisation; sync: the blocking copies
@@ -471,7 +489,15 @@ int main(int argc, char** argv) {
         std::printf("E2E n=%u batch=%u reps=%d e2e_us=%.1f e2e_min_us=%.1f first_us=%.1f boot_us=%.1f plan_us=%.1f "
                     "warmup_us=%.1f mode=%s host=%s input=%s\n",
                     n, batch, ereps, sum / ereps, mn, first, boot_us, plan_us, warm_us,
-                    xfer_only ? "xfer" : use_async ? "async" : "sync", p_in ? "pinned" : "pageable", mode);
+                    pipe ? "pipe" : xfer_only ? "xfer" : use_async ? "async" : "sync", p_in ? "pinned" : "pageable",
+                    mode);
+        if (pipe) {
+            // the pipelined call's last output against the FP64 reference
+            const std::vector<float> got(dst, dst + out_elems);
+            const Errors pe = verify(kind, sign, h_in, got, n, batch);
+            e2e_ok = pe.max_rel <= 1e-4;
+            std::printf("E2E pipe maxRel=%.3e %s\n", pe.max_rel, e2e_ok ? "PASS" : "FAIL");
+        }
         // one logical input and one logical output per execution; the
         // chain's boundary never leaves the device
         std::printf("E2E transfers: in=1 out=1 boundary=0 per_execution\n");
@@ -564,7 +590,7 @@ int main(int argc, char** argv) {
     const char* kname = kind == BFLY_KIND_R2C ? "bfly_r2c" : kind == BFLY_KIND_C2R ? "bfly_c2r" : "bfly_c2c";
     std::printf("%s: %.1f us/call over %d reps (min %.1f us) med %.1f us\n", kname, sum_us / reps, reps, min_us,
                 med_us);
-    const bool pass = err.max_rel <= 1e-4 && seq_ok;
+    const bool pass = err.max_rel <= 1e-4 && seq_ok && e2e_ok;
     std::printf("%s\n", pass ? "PASS" : "FAIL");
 
     for (auto e : ev_chain) (void)bfly_rt_event_destroy(e);

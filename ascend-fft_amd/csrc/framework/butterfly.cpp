@@ -652,19 +652,21 @@ int Plan::execute(int kind, int sign, const float* in, float* out, uint32_t n, u
     const size_t rowIn = kind == BFLY_KIND_C2C ? 2 * (size_t)n : kind == BFLY_KIND_R2C ? n : n + 2;
     const size_t rowOut = kind == BFLY_KIND_C2C ? 2 * (size_t)n : kind == BFLY_KIND_R2C ? n + 2 : n;
     const size_t inB = rowIn * batch * sizeof(float), outB = rowOut * batch * sizeof(float);
-    const size_t ws = workBytes(cand_, impl_->hardware, kind, n, batch);
+    const int variant =
+        kind == BFLY_KIND_C2C ? lowerC2C(cand_, impl_->hardware, n, nullptr) : BFLY_VARIANT_AUTO;
+    // AUTO picks a long C2C's kernel per chunk, which may be the four-step chain
+    size_t ws = workBytes(cand_, impl_->hardware, kind, n, batch);
+    if (kind == BFLY_KIND_C2C && variant == BFLY_VARIANT_AUTO && n > (uint32_t)bfly_plan::kMaxLocalN)
+        ws = (size_t)8 * n * batch;
     if (ensureBuf(&impl_->devIn, &impl_->inCap, inB) || ensureBuf(&impl_->devOut, &impl_->outCap, outB) ||
         (ws && ensureBuf(&impl_->devWork, &impl_->workCap, ws)))
         return -3;
-    if (bfly_rt_memcpy_h2d(impl_->devIn, in, inB, ctxStream_) != BFLY_OK) return -4;
-    const int rc = launchKind(cand_, impl_->hardware, kind, sign, static_cast<const float*>(impl_->devIn),
-                              static_cast<float*>(impl_->devOut), n, batch,
-                              static_cast<const float*>(impl_->devTable[kind]), impl_->devWork, ctxStream_);
-    if (rc == BFLY_ERR_SHAPE) return -8;
-    if (rc != BFLY_OK) return -5;
-    if (bfly_rt_stream_sync(ctxStream_) != BFLY_OK) return -6;
-    if (bfly_rt_memcpy_d2h(out, impl_->devOut, outB, ctxStream_) != BFLY_OK) return -7;
-    return 0;
+    // upload, transform and download pipelined in chunks over the link (the
+    // reference does the three in turn); the library's status codes are the
+    // reference's -1..-8
+    return bfly_host_pipelined(kind, sign, variant, in, out, n, batch,
+                               static_cast<const float*>(impl_->devTable[kind]), impl_->devIn, impl_->devOut,
+                               impl_->devWork, 0, ctxStream_);
 }
 
 int Plan::run(const float* in, float* out, uint32_t n, uint32_t batch) {

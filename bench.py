@@ -20,7 +20,8 @@ the whole batch ([4096][8192] float32, interleaved), inputs resident in HBM.  Pr
                  persistent launch under hipGraph vs per-request launches:
                  transforms/s, GB/s, p50/p99 per-request latency; the
                  spectral-convolution layer (fwd+bwd) vs the same layer on
-                 torch.fft
+                 torch.fft; the host-to-host 4096 x 4096 call over PCIe
+                 (staged vs pipelined, beside the reference's end-to-end row)
 
 Timed steps rotate over several input/output buffer pairs (>= 1 GiB in
 total) so that the 256 MB last-level cache cannot serve one step's input from
@@ -515,6 +516,70 @@ def fft2d_pattern(reps: int = 30):
             "points": points}
 
 
+def host_e2e_pattern(reps: int = 20):
+    """C2C fp32 N=4096 b=4096 from pinned host memory to pinned host memory
+    (the reference's end-to-end row: pinned H2D + FFT + D2H, 7306.3 us on its
+    hardware): the staged upload -> kernel -> download, each step waited for,
+    against bfly_host_pipelined (8 MiB chunks, uploads / transforms /
+    downloads overlapped); wall clock per call, mean / median / min over
+    `reps` after one untimed call; the output of the last pipelined call checked against the
+    staged one bit for bit.  PCIe-inclusive, never the headline value."""
+    import numpy as np
+    import torch
+    from bfly import _lib, ops
+
+    dev = torch.device("cuda:0")
+    lib = _lib.hip()
+    n, b = N, BATCH
+    nbytes = 8 * n * b
+    hin = torch.empty(b, 2 * n, dtype=torch.float32, pin_memory=True)
+    hin.copy_(torch.from_numpy(np.random.default_rng(7).uniform(-1, 1, (b, 2 * n)).astype(np.float32)))
+    hout = torch.empty(b, 2 * n, dtype=torch.float32, pin_memory=True)
+    hout2 = torch.empty(b, 2 * n, dtype=torch.float32, pin_memory=True)
+    din = torch.empty(b * 2 * n, dtype=torch.float32, device=dev)
+    dout = torch.empty_like(din)
+    tab = ops.table(_lib.KIND_C2C, n, dev)
+    s = torch.cuda.current_stream(dev).cuda_stream
+    variant = _lib.VARIANT_STREAM_LT
+
+    def staged():
+        _lib.check(lib.bfly_rt_memcpy_h2d(din.data_ptr(), hin.data_ptr(), nbytes, s), "h2d")
+        _lib.check(lib.bfly_c2c_ex(din.data_ptr(), dout.data_ptr(), n, b, -1, variant, tab.data_ptr(), None, s),
+                   "bfly_c2c_ex")
+        _lib.check(lib.bfly_rt_stream_sync(s), "sync")
+        _lib.check(lib.bfly_rt_memcpy_d2h(hout.data_ptr(), dout.data_ptr(), nbytes, s), "d2h")
+
+    def pipelined():
+        _lib.check(lib.bfly_host_pipelined(_lib.KIND_C2C, -1, variant, hin.data_ptr(), hout2.data_ptr(), n, b,
+                                           tab.data_ptr(), din.data_ptr(), dout.data_ptr(), None, 0, s),
+                   "bfly_host_pipelined")
+
+    def timed(fn):
+        fn()
+        ts = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            fn()
+            ts.append((time.perf_counter() - t0) * 1e6)
+        return float(np.mean(ts)), float(np.median(ts)), float(np.min(ts))
+
+    torch.cuda.synchronize()
+    st_mean, st_med, st_min = timed(staged)
+    pl_mean, pl_med, pl_min = timed(pipelined)
+    same = bool(torch.equal(hout, hout2))
+    ref_us = 7306.3
+    return {"workload": "C2C fp32 N=4096 b=4096, pinned host in -> pinned host out (128 MiB each way), "
+                        "wall clock per call",
+            "staged_us": round(st_mean, 1), "staged_median_us": round(st_med, 1), "staged_min_us": round(st_min, 1),
+            "pipelined_us": round(pl_mean, 1), "pipelined_median_us": round(pl_med, 1),
+            "pipelined_min_us": round(pl_min, 1),
+            "pipelined_GB_per_s_both_ways": round(2 * nbytes / pl_mean / 1e3, 1),
+            "pipelined_equals_staged": same,
+            "reference_e2e_us": ref_us, "reference_source": "published end-to-end row, Ascend910 (pinned H2D + "
+                                                            "FFT + D2H)",
+            "speedup_vs_reference": round(ref_us / pl_mean, 2)}
+
+
 def workload_loop(name: str, steps: int, warmup: int):
     """One secondary workload run `steps` times after `warmup` untimed runs,
     for kernel traces and counter passes of its kernels (--only NAME).
@@ -736,7 +801,8 @@ def main():
     if not args.no_patterns:
         line["rows"] = secondary_rows(0 if args.no_cpu else 1.5)
         line["patterns"] = {"grouped_uneven": grouped_pattern(), "spectral_conv1d": spectral_pattern(),
-                            "grid49_vs_torch_fft": grid49_pattern(), "fft2d_vs_torch_fft2": fft2d_pattern()}
+                            "grid49_vs_torch_fft": grid49_pattern(), "fft2d_vs_torch_fft2": fft2d_pattern(),
+                            "host_end_to_end": host_e2e_pattern()}
     print(json.dumps(line))
 
 

@@ -98,6 +98,35 @@ BFLY_API int bfly_rt_time_transform(int kind, int sign, const float* in, float* 
                            uint32_t batch, const float* table, void* work, int reps,
                            void* stream, float* ms);
 
+/* Host-to-host transform of host buffers, pipelined over the PCIe link: the
+ * batch is cut into chunks of `chunk_rows` transforms (0: about 8 MiB of
+ * input per chunk) and chunk k's upload (on `stream`), transform and
+ * download (on two streams of the library's own, created once per device by
+ * bfly_rt_init or the first call) are ordered by events, so uploads,
+ * transforms and downloads of different chunks overlap and the full-duplex
+ * link carries both directions at once.  A batch of one chunk runs its
+ * three steps on `stream` alone.
+ * Replaces the synchronous upload -> kernel -> download of the reference's
+ * Plan::run / runR2C / runC2R (src/framework/butterfly.cpp:723-850) for
+ * callers holding host data; returns once the result is in `host_out`.
+ *   kind BFLY_KIND_C2C: sign -1 / +1, unnormalised, rows of 2n floats;
+ *   BFLY_KIND_R2C: [batch][n] -> [batch][n+2]; BFLY_KIND_C2R:
+ *   [batch][n+2] -> [batch][n] scaled 1/n (full spectra; sign ignored).
+ * Every chunk runs one kernel variant: `variant` as for bfly_c2c_ex /
+ * bfly_real_ex, BFLY_VARIANT_AUTO meaning the AUTO choice for a chunk's
+ * batch, so the result equals bfly_c2c_ex / bfly_real_ex of that variant on
+ * the whole batch bit for bit.  `dev_in` / `dev_out` hold the whole batch
+ * on the device; `work` is the long-C2C workspace of one chunk
+ * (bfly_workspace_bytes(kind, n, chunk rows)) or NULL when it needs none.
+ * Host memory may be pinned (bfly_rt_malloc_host) or pageable; only pinned
+ * memory lets the two directions overlap.  Calls on one device serialise
+ * on the library's streams.  Status codes as for the other entry points
+ * (BFLY_ERR_H2D / _LAUNCH / _D2H / _SYNC / _SHAPE); after an error every
+ * issued step has finished before the call returns. */
+BFLY_API int bfly_host_pipelined(int kind, int sign, int variant, const float* host_in, float* host_out,
+                                 uint32_t n, uint32_t batch, const float* table, void* dev_in, void* dev_out,
+                                 void* work, uint32_t chunk_rows, void* stream);
+
 /* ------------------------------------------------------------- hipGraph
  * Capture of the device-pointer calls issued on `stream` into a replayable
  * graph: bfly_graph_begin starts capturing the stream (thread-local mode;

@@ -374,3 +374,70 @@ def test_python_modules_import_and_variant_ids_match_header():
     for name in ("FORWARD", "GRAD_INPUT", "GRAD_WEIGHT"):
         m = re.search(rf"#define BFLY_MIX_{name} (\d+)", text)
         assert m and int(m.group(1)) == getattr(_lib, f"MIX_{name}"), name
+
+
+def test_host_pipelined_chunks_and_guards_through_stub():
+    """bfly_host_pipelined on the CPU stub: the batch moves in chunks of
+    chunk_rows transforms (upload, launch, download per chunk, in order),
+    the whole batch is guarded as one execute call first, and an injected
+    failure stops the call with the step's code."""
+    s = bfly.stub()
+    s.bfly_stub_reset()
+    n, b = 64, 5
+    host_in = np.arange(b * 2 * n, dtype=np.float32)
+    host_out = np.zeros_like(host_in)
+    dev_in = np.zeros_like(host_in)
+    dev_out = np.full_like(host_in, 7.0)  # the stub's launch computes nothing
+    tab = np.zeros(16, np.float32)
+    hi, ho, di, do, tp = (a.ctypes.data for a in (host_in, host_out, dev_in, dev_out, tab))
+    cnt = (C.c_int * 8)()
+
+    def counters():
+        s.bfly_stub_counters(cnt)
+        return cnt[2], cnt[3]  # copies, launches
+
+    c0, l0 = counters()
+    assert s.bfly_host_pipelined(bfly.KIND_C2C, -1, 0, hi, ho, n, b, tp, di, do, None, 2, None) == 0
+    c1, l1 = counters()
+    assert (c1 - c0, l1 - l0) == (6, 3)  # three chunks of 2, 2, 1 transforms
+    assert np.array_equal(dev_in, host_in) and np.all(host_out == 7.0)
+    # the default chunk (about 8 MiB of input) holds this whole batch
+    assert s.bfly_host_pipelined(bfly.KIND_C2C, 1, 0, hi, ho, n, b, tp, di, do, None, 0, None) == 0
+    c2, l2 = counters()
+    assert (c2 - c1, l2 - l1) == (2, 1)
+    # the second chunk's upload fails: H2D, one launch done
+    s.bfly_stub_set_failures(0, c2 + 3, 0)
+    assert s.bfly_host_pipelined(bfly.KIND_C2C, -1, 0, hi, ho, n, b, tp, di, do, None, 2, None) == _lib.ERR_H2D
+    s.bfly_stub_set_failures(0, 0, 0)
+    c3, l3 = counters()
+    assert (c3 - c2, l3 - l2) == (3, 1)
+    # the first download fails: D2H
+    s.bfly_stub_set_failures(0, c3 + 2, 0)
+    assert s.bfly_host_pipelined(bfly.KIND_C2C, -1, 0, hi, ho, n, b, tp, di, do, None, 2, None) == _lib.ERR_D2H
+    s.bfly_stub_set_failures(0, 0, 0)
+    # real kinds: rows of n reals and n+2 floats
+    r_in = np.zeros(b * n, np.float32); r_half = np.zeros(b * (n + 2), np.float32)
+    d_in = np.zeros_like(r_in); d_half = np.zeros_like(r_half)
+    args = (r_in.ctypes.data, r_half.ctypes.data, n, b, tp, d_in.ctypes.data, d_half.ctypes.data, None, 2, None)
+    assert s.bfly_host_pipelined(bfly.KIND_R2C, 0, 0, *args) == 0  # the sign of a real kind is implied
+    args = (r_half.ctypes.data, r_in.ctypes.data, n, b, tp, d_half.ctypes.data, d_in.ctypes.data, None, 2, None)
+    assert s.bfly_host_pipelined(bfly.KIND_C2R, 0, 0, *args) == 0
+    # guards (nothing is copied or launched)
+    c4, l4 = counters()
+    bad = [
+        (7, -1, 0, hi, ho, n, b, tp, di, do, None, 2, None),                      # kind
+        (bfly.KIND_C2C, 0, 0, hi, ho, n, b, tp, di, do, None, 2, None),           # sign
+        (bfly.KIND_C2C, -1, 0, None, ho, n, b, tp, di, do, None, 2, None),        # host input
+        (bfly.KIND_C2C, -1, 0, hi, ho, n, b, tp, None, do, None, 2, None),        # device output
+        (bfly.KIND_C2C, -1, 0, hi, ho, 48, b, tp, di, do, None, 2, None),         # length
+        (bfly.KIND_C2C, -1, 0, hi, ho, n, 0, tp, di, do, None, 2, None),          # batch
+        (bfly.KIND_C2C, -1, _lib.VARIANT_LAST + 1, hi, ho, n, b, tp, di, do, None, 2, None),
+        (bfly.KIND_R2C, 0, _lib.VARIANT_STREAM2, hi, ho, n, b, tp, di, do, None, 2, None),
+        (bfly.KIND_C2C, -1, 0, hi, ho, n, b, tp, di, di, None, 2, None),          # device buffers overlap
+        (bfly.KIND_C2C, -1, 0, hi, ho, 32768, 1, tp, di, do, None, 2, None),      # four-step needs work
+    ]
+    for a in bad:
+        assert s.bfly_host_pipelined(*a) == bfly.ERR_SHAPE, a[:3]
+    assert s.bfly_host_pipelined(bfly.KIND_C2C, -1, 0, hi, ho, n, b, None, di, do, None, 2, None) == _lib.ERR_TABLE
+    assert counters() == (c4, l4)
+

@@ -146,6 +146,11 @@ def test_report_lines_follow_the_vocabulary(cli):
     r = run(cli, 4096, 2, 2, AB_E2E=3, AB_E2E_MODE="xfer")
     assert "E2E n=4096 batch=2 reps=3" in r.stdout and "mode=xfer host=pinned" in r.stdout
     assert "E2E transfers: in=1 out=1 boundary=0 per_execution" in r.stdout
+    # the pipelined host-to-host call checks its own output: the stub computes
+    # nothing, so the check fails and the program stops with status 1
+    r = run(cli, 1024, 5, 1, AB_E2E=2, AB_E2E_MODE="pipe", AB_E2E_CHUNK=2)
+    assert "E2E n=1024 batch=5 reps=2" in r.stdout and "mode=pipe host=pinned" in r.stdout
+    assert "E2E pipe maxRel=" in r.stdout and "FAIL" in r.stdout and r.returncode == 1
 
 
 # ---------------------------------------------------------------- reader

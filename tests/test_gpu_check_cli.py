@@ -108,12 +108,18 @@ def test_input_sequence_reuses_the_plan_without_stale_outputs(n, batch):
 
 
 @pytest.mark.parametrize("n,mode,host", [(4096, "async", "pinned"), (4096, "sync", "pageable"),
-                                         (4096, "xfer", "pinned"), (65536, "async", "pinned")])
+                                         (4096, "xfer", "pinned"), (65536, "async", "pinned"),
+                                         (4096, "pipe", "pinned"), (16384, "pipe", "pageable"),
+                                         (65536, "pipe", "pinned")])
 def test_end_to_end_lines(n, mode, host):
-    r = run(n, 64 if n <= 4096 else 4, 3, AB_E2E=5, AB_E2E_MODE=mode, AB_E2E_HOST=host)
+    # pipe: bfly_host_pipelined in chunks of 3 transforms (a ragged last one),
+    # its output checked against FP64 by the program itself
+    r = run(n, 64 if n <= 4096 else 4, 3, AB_E2E=5, AB_E2E_MODE=mode, AB_E2E_HOST=host, AB_E2E_CHUNK=3)
     assert r.returncode == 0, r.stdout + r.stderr
     line = next(l for l in r.stdout.splitlines() if l.startswith("E2E n="))
     assert f"mode={mode} host={host}" in line
     e2e = float(line.split("e2e_us=")[1].split()[0])
     assert e2e > 0
     assert "E2E transfers: in=1 out=1 boundary=0 per_execution" in r.stdout
+    if mode == "pipe":
+        assert next(l for l in r.stdout.splitlines() if l.startswith("E2E pipe maxRel=")).endswith("PASS")
