@@ -83,9 +83,11 @@ def _gate_c2c(got, x, n, sign):
 
 
 # (n, batch, chunk rows, variant, sign): ragged last chunks, one chunk, the
-# default chunk, the single-workgroup and four-step long kernels
+# default chunk (1024 transforms of 1024 points: five chunks for 4100), the
+# single-workgroup and four-step long kernels
 C2C_CASES = [
     (1024, 100, 7, 5, -1),
+    (1024, 4100, 0, 5, -1),
     (4096, 300, 0, 5, 1),
     (4096, 33, 64, 1, -1),
     (256, 4097, 1000, 2, -1),
