@@ -25,7 +25,6 @@ int launched(hipError_t e) { return e == hipSuccess ? BFLY_OK : BFLY_ERR_LAUNCH;
 // only after the wait that used its previous recording was issued.
 constexpr int kPipeRing = 32;
 constexpr int kPipeDevices = 64;
-constexpr size_t kPipeChunkBytes = 8u << 20;
 struct PipeStreams {
     hipStream_t comp = nullptr, down = nullptr;
     hipEvent_t ev[2 * kPipeRing] = {};
@@ -55,11 +54,10 @@ int pipe_ready(PipeStreams& P) {
         }
     P.ready = true;
     // a first kernel on the transform stream, a first recording of every
-    // event and a first download on the download stream, so that their
+    // event and a first download on the download stream (large enough to go
+    // through the copy engine: small copies run as kernels), so that their
     // set-up is paid here rather than in the first call (best effort: a
     // failure leaves that set-up to the first call)
-    // (the download is large enough to go through the copy engine: small
-    // copies run as kernels)
     constexpr size_t kWarm = 4u << 20;
     void *d = nullptr, *h = nullptr;
     if (hipMalloc(&d, kWarm) == hipSuccess && hipHostMalloc(&h, kWarm, hipHostMallocDefault) == hipSuccess) {
