@@ -289,15 +289,56 @@ __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], floa
 #pragma unroll
         for (int r = 0; r < RL; r++) L[lds_idx(t + m * T + r * (M / RL))] = v[m * RL + r];
     __syncthreads();
+    if constexpr (T >= 64) {
+        // Bins j and M - j from one pair of reads: with E, O the even / odd
+        // spectra at j, X[j] = E + W_n^j O and X[M-j] = conj(E - W_n^j O)
+        // (E and O at M - j are the conjugates of those at j, W_n^{M-j} =
+        // -conj W_n^j), so each thread reads half the bins and one twiddle
+        // per pair.  j = M/2 is its own mirror and is stored once.  Measured
+        // (MI355X, LDS twiddles): 4096 x 4096 27.9 -> 26.9 us, 8192 x 2048
+        // 28.9 -> 27.0, 2048 x 8192 28.2 -> 27.7; at T = 32 the mirrored
+        // stores, which cannot be line-aligned together with the ascending
+        // ones, made 1024 x 16384 slower (31.9 against 27.6 us below).
+        if (tr < batch) {
+            float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
+            constexpr int H = M / 2;
+#pragma unroll
+            for (int i = 0; i <= H / T; i++) {
+                const int j = t + i * T;
+                // truncated spectra (kout <= M) skip the pairs they do not store
+                if (j <= H && (j < (int)kout || M - j < (int)kout)) {
+                    const float2 a = L[lds_idx(j & (M - 1))];
+                    const float2 b = L[lds_idx((M - j) & (M - 1))];
+                    const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
+                    const float2 o = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
+                    const float2 wo = cmul_tw<-1>(o, tws.w(j));
+                    float2 x = cadd(e, wo);
+                    float2 y = make_float2(e.x - wo.x, wo.y - e.y);
+                    if (adj) {
+                        const float f = 1.0f / (float)M;
+                        if (j == 0) {
+                            x = make_float2(x.x * (0.5f * f), 0.f);
+                            y = make_float2(y.x * (0.5f * f), 0.f);
+                        } else {
+                            x = make_float2(x.x * f, x.y * f);
+                            y = make_float2(y.x * f, y.y * f);
+                        }
+                    }
+                    if (j < (int)kout) st_io<true>(&dst[j], x);
+                    if (j != H && M - j < (int)kout) st_io<true>(&dst[M - j], y);
+                }
+            }
+        }
+        return;
+    }
     if (tr < batch) {
         float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
         // Rows of n+2 floats start 8 bytes further into a 128-byte line each
-        // time.  Where a wave's store covers only a few lines (T <= 64), the
-        // bins are shifted by the row's offset in its line, so that every
+        // time.  Where a wave's store covers only a few lines (T <= 32 here),
+        // the bins are shifted by the row's offset in its line, so that every
         // store starts on a line boundary (same bins, same arithmetic, other
-        // lanes): R2C 1024 x 16384 31.5 -> 27.7 us, 2048 x 8192 28.2 -> 27.9;
-        // at T = 128 it measured neutral to slightly slower (4096 x 4096).
-        const int s = (T >= 16 && T <= 64) ? (int)((reinterpret_cast<uintptr_t>(dst) >> 3) & 15) : 0;
+        // lanes): R2C 1024 x 16384 31.5 -> 27.7 us.
+        const int s = T >= 16 ? (int)((reinterpret_cast<uintptr_t>(dst) >> 3) & 15) : 0;
 #pragma unroll
         for (int i = 0; i <= P; i++) {
             const int k = t + i * T - s;

@@ -27,7 +27,7 @@ def _host(shape, pinned):
     return t, t.numpy()
 
 
-def _pipelined(kind, sign, variant, x, n, b, row_out, chunk, pinned, work_rows=0):
+def _pipelined(kind, sign, variant, x, n, b, row_out, chunk, pinned, work_rows=0, null_stream=False):
     import torch
     from bfly import _lib, ops
     lib = _lib.hip()
@@ -43,7 +43,7 @@ def _pipelined(kind, sign, variant, x, n, b, row_out, chunk, pinned, work_rows=0
     torch.cuda.synchronize()
     rc = lib.bfly_host_pipelined(kind, sign, variant, tin.data_ptr(), tout.data_ptr(), n, b, tab.data_ptr(),
                                  din.data_ptr(), dout.data_ptr(), None if work is None else work.data_ptr(),
-                                 chunk, torch.cuda.current_stream().cuda_stream)
+                                 chunk, None if null_stream else torch.cuda.current_stream().cuda_stream)
     assert rc == 0, f"bfly_host_pipelined kind={kind} n={n} b={b} chunk={chunk} rc={rc}"
     return hout.copy()
 
@@ -134,6 +134,19 @@ def test_pipelined_real_equals_the_whole_batch_call(n, b, chunk, variant, pinned
     back = _pipelined(_lib.KIND_C2R, 0, variant, spec, n, b, n, chunk, pinned)
     assert np.array_equal(back, _whole_real(_lib.KIND_C2R, spec, n, b, variant))
     assert oracle.rel_l2_rows(back, x) <= REL_L2  # C2R scales 1/n: the round trip is the input
+
+
+@pytest.mark.parametrize("chunk", [0, 24])
+def test_pipelined_on_the_null_stream(chunk):
+    # uploads on the legacy default stream, transforms and downloads on the
+    # library's own streams: one chunk and seven chunks
+    from bfly import _lib
+    print(f"pipelined c2c on the null stream chunk={chunk}", flush=True)
+    n, b = 2048, 150
+    x = oracle.uniform_input(34, (b, 2 * n))
+    got = _pipelined(_lib.KIND_C2C, -1, 5, x, n, b, 2 * n, chunk, True, null_stream=True)
+    assert np.array_equal(got, _whole_c2c(x, n, b, -1, 5))
+    _gate_c2c(got, x, n, -1)
 
 
 def test_pipelined_guards_return_before_any_copy():
