@@ -205,12 +205,14 @@ def test_real_streaming_variant_equals_one_shot(n):
 @pytest.mark.parametrize("n", [256, 1024, 4096, 8192])
 def test_real_lds_twiddle_variant_fp64(n):
     # bfly_real_ex STREAM_LT: twiddles from two-level LDS tables (different
-    # rounding from the one-shot kernel); FP64 gates, round trip, truncation.
+    # rounding from the one-shot kernel); FP64 gates, round trip, truncation
+    # (13 bins, and 3n/8 + 5 bins: from n = 2048 bins j and n/2 - j are
+    # formed together, so a cut above n/4 keeps some mirrored bins).
     import torch
     from bfly import _lib
     lib = _lib.hip()
     s = torch.cuda.current_stream().cuda_stream
-    for b, bins in ((1, n // 2 + 1), (777, n // 2 + 1), (3000, 13)):
+    for b, bins in ((1, n // 2 + 1), (777, n // 2 + 1), (3000, 13), (101, 3 * n // 8 + 5)):
         x = oracle.uniform_input(33, (b, n))
         xt = torch.from_numpy(x).cuda()
         y = torch.full((b, 2 * bins), float("nan"), device="cuda")

@@ -94,6 +94,7 @@ C2C_CASES = [
     (16384, 70, 16, 7, -1),
     (16384, 40, 9, 5, 1),
     (65536, 5, 2, 7, -1),
+    (32768, 6, 4, 7, 1),
 ]
 
 
