@@ -299,14 +299,23 @@ __device__ __forceinline__ void r2c_finish(float2 (&v)[bfly_plan::kPoints], floa
         // 28.9 -> 27.0, 2048 x 8192 28.2 -> 27.7; at T = 32 the mirrored
         // stores, which cannot be line-aligned together with the ascending
         // ones, made 1024 x 16384 slower (31.9 against 27.6 us below).
+        // The one-shot and streaming forms share this code, so they stay
+        // bit-equal to each other.
         if (tr < batch) {
             float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)ostride);
             constexpr int H = M / 2;
+            // the ascending stores start on a 128-byte line (j shifted by the
+            // row's offset in its line, as below; the mirrored ones cannot
+            // be aligned at the same time): 2048 x 8192 27.5 -> 26.7 us,
+            // 4096 x 4096 27.1 -> 26.3; at T = 256 (8192 x 2048) it measured
+            // 27.0 -> 27.3 and is left out
+            constexpr bool kShift = T <= 128;
+            const int s = kShift ? (int)((reinterpret_cast<uintptr_t>(dst) >> 3) & 15) : 0;
 #pragma unroll
-            for (int i = 0; i <= H / T; i++) {
-                const int j = t + i * T;
+            for (int i = 0; i <= H / T + (kShift ? 1 : 0); i++) {
+                const int j = t + i * T - s;
                 // truncated spectra (kout <= M) skip the pairs they do not store
-                if (j <= H && (j < (int)kout || M - j < (int)kout)) {
+                if (j >= 0 && j <= H && (j < (int)kout || M - j < (int)kout)) {
                     const float2 a = L[lds_idx(j & (M - 1))];
                     const float2 b = L[lds_idx((M - j) & (M - 1))];
                     const float2 e = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
