@@ -118,8 +118,9 @@ BFLY_API int bfly_rt_time_transform(int kind, int sign, const float* in, float* 
  * the whole batch bit for bit.  `dev_in` / `dev_out` hold the whole batch
  * on the device; `work` is the long-C2C workspace of one chunk
  * (bfly_workspace_bytes(kind, n, chunk rows)) or NULL when it needs none.
- * Host memory may be pinned (bfly_rt_malloc_host) or pageable; only pinned
- * memory lets the two directions overlap.  Calls on one device serialise
+ * Host memory may be pinned (bfly_rt_malloc_host) or pageable; pageable
+ * buffers (the runtime stages their copies itself, one direction at a time)
+ * run the three steps in turn on `stream`, as one chunk.  Calls on one device serialise
  * on the library's streams.  Status codes as for the other entry points
  * (BFLY_ERR_H2D / _LAUNCH / _D2H / _SYNC / _SHAPE); after an error every
  * issued step has finished before the call returns. */
