@@ -414,7 +414,17 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream(const float* _
         int t2 = tid;  // opaque per iteration (see k_r2c_stream)
         asm volatile("" : "+v"(t2));
         if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
-        c2r_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, g, lds, t2, adj != 0);
+        if constexpr (PlanOf<I>::T >= 16) {
+            // the same steps as c2r_finish through its two halves, which the
+            // compiler schedules differently: 512 x 32768 32.3 -> 30.2 us
+            // (and 256 x 65536 30.2 -> 34 us, hence only from 16 threads)
+            c2r_stage<I, bfly_plan::kBlock>(cur, lds, t2);
+            __syncthreads();
+            c2r_compute<I, bfly_plan::kBlock>(out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, g, lds, t2, adj != 0);
+        } else {
+            c2r_finish<I, bfly_plan::kBlock>(cur, out, RealTwGlobal<I, 0>{tw, wn}, batch, scale, g, lds, t2,
+                                             adj != 0);
+        }
         __syncthreads();  // the last exchange's reads end before the next staging writes
 #pragma unroll
         for (int i = 0; i <= P; i++) cur[i] = nxt[i];
@@ -454,7 +464,8 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_r2c_stream_lt(const float
 }
 
 template <int I>
-__global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float* __restrict__ in,
+__global__ __launch_bounds__(bfly_plan::kBlock) __attribute__((amdgpu_waves_per_eu(PlanOf<I>::N >= 4096 ? 4 : 1))) void
+k_c2r_stream_lt(const float* __restrict__ in,
                                                                      float* __restrict__ out,
                                                                      const float2* __restrict__ tw,
                                                                      const float2* __restrict__ wn,
@@ -474,8 +485,22 @@ __global__ __launch_bounds__(bfly_plan::kBlock) void k_c2r_stream_lt(const float
         const uint32_t gn = g + gridDim.x;
         int t2 = tid;  // opaque per iteration (see k_r2c_stream)
         asm volatile("" : "+v"(t2));
-        if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
-        c2r_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, scale, g, lds, t2, adj != 0);
+        if constexpr (M >= 4096) {
+            // Stage the current rows before the next rows' loads are issued:
+            // the wait that the staging needs (for the current rows' loads,
+            // issued a step earlier) then no longer holds back the prefetch,
+            // which stays in flight through the pre-twiddle and the FFT; with
+            // the kernel held to 128 VGPRs (four workgroups per CU) n = 8192
+            // x 2048 measured 30.7 -> 27.5-28.5 us.  At n <= 4096 it measured
+            // flat (2048, 4096) or slower (512: 29.3 -> 30.1 us).
+            c2r_stage<I, bfly_plan::kBlock, 0>(cur, lds, t2);
+            __syncthreads();
+            if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
+            c2r_compute<I, bfly_plan::kBlock, 0>(out, tws, batch, scale, g, lds, t2, adj != 0);
+        } else {
+            if (gn < groups) c2r_load<I, bfly_plan::kBlock>(nxt, in, batch, kin, istride, gn, t2);
+            c2r_finish<I, bfly_plan::kBlock, 0>(cur, out, tws, batch, scale, g, lds, t2, adj != 0);
+        }
         __syncthreads();
 #pragma unroll
         for (int i = 0; i <= P; i++) cur[i] = nxt[i];

@@ -397,12 +397,95 @@ __device__ __forceinline__ void c2r_load(float2 (&q)[bfly_plan::kPoints + 1], co
     const int bl = tid / T, t = tid % T;
     const uint64_t tr = (uint64_t)g * B + bl;
     const bool active = tr < batch;
+    static_assert(P * T == M, "bins t + i T, i < P, cover 0 .. M-1");
     const float2* src = reinterpret_cast<const float2*>(in + tr * (uint64_t)istride);
+    if constexpr (T < 16) {
+        // short rows: bin M loaded as the others
+#pragma unroll
+        for (int i = 0; i <= P; i++) {
+            const int k = t + i * T;
+            q[i] = (active && k <= M && k < (int)kin) ? ld_io<false>(&src[k]) : make_float2(0.f, 0.f);
+        }
+        return;
+    }
+#pragma unroll
+    for (int i = 0; i < P; i++) {
+        const int k = t + i * T;
+        q[i] = (active && k < (int)kin) ? ld_io<true>(&src[k]) : make_float2(0.f, 0.f);
+    }
+    // bin M (thread 0 only): its imaginary part is never read (the
+    // pre-twiddle zeroes it), so only the real part is loaded and held.  The
+    // other threads read the last bin of their own row instead (an input
+    // row past the batch reads the first row) and drop it: no branch around
+    // the load, whose join would make the compiler wait for every load in
+    // flight, the prefetch included.  Measured with the AUTO kernels: C2R
+    // 1024 x 16384 27.6 -> 26.8 us; it also brings the 8192-point
+    // LDS-twiddle kernel within 128 VGPRs without spilling (four
+    // workgroups per CU).
+    const int kM = t + P * T;
+    const bool okM = active && kM <= M && kM < (int)kin;
+    const int kLast = (int)kin - 1 < M ? (int)kin - 1 : M;
+    const float* fM = reinterpret_cast<const float*>(active ? &src[okM ? kM : kLast] : reinterpret_cast<const float2*>(in));
+    const float xm = __builtin_nontemporal_load(fM);
+    q[P] = make_float2(okM ? xm : 0.f, 0.f);
+}
+
+// The two halves of c2r_finish, for a caller that issues other work (the
+// next rows' loads) between them.  Staging: the thread's bins into its
+// transform's LDS region.
+template <int I, int WG, int SC = 0>
+__device__ __forceinline__ void c2r_stage(const float2 (&q)[bfly_plan::kPoints + 1], float2* lds, int tid) {
+    using PL = PlanOf<I, SC>;
+    constexpr int M = PL::N, T = PL::T, P = PL::P;
+    static_assert(lds_len(M) >= M + 1, "staging needs M+1 bins");
+    static_assert(WG % T == 0, "a work unit holds whole transforms");
+    const int bl = tid / T, t = tid % T;
+    float2* L = lds + bl * lds_len(M);
 #pragma unroll
     for (int i = 0; i <= P; i++) {
         const int k = t + i * T;
-        q[i] = (active && k <= M && k < (int)kin) ? ld_io<(T >= 16)>(&src[k]) : make_float2(0.f, 0.f);
+        if (k <= M) L[k] = q[i];
     }
+}
+
+// The rest of the compute phase, after the staging has been synchronised:
+// pre-twiddle from the staged bins, inverse FFT_M, reals scaled and stored.
+template <int I, int WG, int SC = 0, class TWS = RealTwGlobal<I, SC>>
+__device__ __forceinline__ void c2r_compute(float* __restrict__ out, const TWS& tws, uint32_t batch, float scale,
+                                            uint32_t g, float2* lds, int tid, bool adj) {
+    using PL = PlanOf<I, SC>;
+    constexpr int M = PL::N, T = PL::T, B = WG / T, R0 = PL::R0, RL = PL::RL, P = PL::P;
+    const int bl = tid / T, t = tid % T;
+    const uint64_t tr = (uint64_t)g * B + bl;
+    float2* L = lds + bl * lds_len(M);
+    float2 v[P];
+#pragma unroll
+    for (int m = 0; m < P / R0; m++)
+#pragma unroll
+        for (int r = 0; r < R0; r++) {
+            const int k = t + m * T + r * (M / R0);
+            float2 a = L[k], b = L[M - k];
+            if (k == 0) {
+                a.y = 0.f;
+                b.y = 0.f;
+                if (adj) { a.x *= 2.f; b.x *= 2.f; }
+            }
+            const float2 s = make_float2(a.x + b.x, a.y - b.y);
+            const float2 d = make_float2(a.x - b.x, a.y + b.y);
+            const float2 wd = cmul_tw<1>(d, tws.w(k));  // W_n^{-k} d
+            v[m * R0 + r] = make_float2(s.x - wd.y, s.y + wd.x);
+        }
+    __syncthreads();
+    tws.template fft<T, 1>(v, L, t);
+    if (tr >= batch) return;
+    float2* dst = reinterpret_cast<float2*>(out + tr * (uint64_t)(2 * M));
+#pragma unroll
+    for (int m = 0; m < P / RL; m++)
+#pragma unroll
+        for (int r = 0; r < RL; r++) {
+            const float2 z = v[m * RL + r];
+            st_io<(T >= 16)>(&dst[t + m * T + r * (M / RL)], make_float2(z.x * scale, z.y * scale));
+        }
 }
 
 // Compute phase: bins staged in LDS, Hermitian pre-twiddle, inverse FFT_M,
