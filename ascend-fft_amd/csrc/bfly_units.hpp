@@ -422,12 +422,15 @@ __device__ __forceinline__ void c2r_load(float2 (&q)[bfly_plan::kPoints + 1], co
     // 1024 x 16384 27.6 -> 26.8 us; it also brings the 8192-point
     // LDS-twiddle kernel within 128 VGPRs without spilling (four
     // workgroups per CU).
-    const int kM = t + P * T;
-    const bool okM = active && kM <= M && kM < (int)kin;
-    const int kLast = (int)kin - 1 < M ? (int)kin - 1 : M;
-    const float* fM = reinterpret_cast<const float*>(active ? &src[okM ? kM : kLast] : reinterpret_cast<const float2*>(in));
-    const float xm = __builtin_nontemporal_load(fM);
-    q[P] = make_float2(okM ? xm : 0.f, 0.f);
+    // A truncated input (kin <= M, uniform) has no bin M and skips the load.
+    q[P] = make_float2(0.f, 0.f);
+    if ((int)kin > M) {
+        const int kM = t + P * T;
+        const bool okM = active && kM == M;
+        const float* fM = reinterpret_cast<const float*>(active ? &src[okM ? kM : M] : reinterpret_cast<const float2*>(in));
+        const float xm = __builtin_nontemporal_load(fM);
+        q[P].x = okM ? xm : 0.f;
+    }
 }
 
 // The two halves of c2r_finish, for a caller that issues other work (the
