@@ -108,7 +108,7 @@ def test_four_step_chain_equals_direct_fft():
 
 
 def test_long_split_rule():
-    # src/host/fft_check.cpp:295-308 splits (SURVEY call stack 5).
+    # the four-step splits of the long chain, src/host/fft_check.cpp:295-308
     assert oracle.long_split(8192) == (64, 128)
     assert oracle.long_split(16384) == (128, 128)
     assert oracle.long_split(32768) == (128, 256)
