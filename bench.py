@@ -684,7 +684,7 @@ def workload_loop(name: str, steps: int, warmup: int):
 def main():
     args = parse()
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on exactly one GPU; --gpus N>1 is not "
+        print(json.dumps({"error": "this benchmark runs on exactly one GPU; --gpus N>1 is not "
                                    "supported (batch rows would shard with no communication)"}))
         sys.exit(2)
     if args.only:
