@@ -166,3 +166,41 @@ def test_pipelined_guards_return_before_any_copy():
                                    d.data_ptr(), e.data_ptr(), None, 0, stream) == _lib.ERR_SHAPE
     assert lib.bfly_host_pipelined(_lib.KIND_C2C, -1, 0, h.ctypes.data, h.ctypes.data, n, b, tab.data_ptr(),
                                    d.data_ptr(), d.data_ptr(), None, 0, stream) == _lib.ERR_SHAPE
+
+
+def test_pipelined_calls_from_four_threads_at_once():
+    # four host threads (ctypes releases the GIL during the call) pipelining
+    # their own buffers on one device: the library's streams are shared under
+    # its lock, and every result equals the whole-batch call
+    import threading
+    import torch
+    from bfly import _lib, ops
+    lib = _lib.hip()
+    n, b, chunk = 1024, 2048, 256
+    dev = torch.device("cuda:0")
+    tab = ops.table(_lib.KIND_C2C, n, dev)
+    jobs = []
+    for i in range(4):
+        x = oracle.uniform_input(40 + i, (b, 2 * n))
+        tin, hin = _host(x.shape, True)
+        hin[...] = x
+        tout, hout = _host(x.shape, True)
+        din = torch.empty(x.size, dtype=torch.float32, device=dev)
+        dout = torch.empty_like(din)
+        jobs.append((x, tin, tout, hout, din, dout))
+    torch.cuda.synchronize()
+    rcs = [None] * 4
+
+    def run(i):
+        x, tin, tout, hout, din, dout = jobs[i]
+        rcs[i] = lib.bfly_host_pipelined(_lib.KIND_C2C, -1, 5, tin.data_ptr(), tout.data_ptr(), n, b,
+                                         tab.data_ptr(), din.data_ptr(), dout.data_ptr(), None, chunk, None)
+
+    threads = [threading.Thread(target=run, args=(i,)) for i in range(4)]
+    for th in threads:
+        th.start()
+    for th in threads:
+        th.join(timeout=60)
+    assert rcs == [0, 0, 0, 0]
+    for x, tin, tout, hout, din, dout in jobs:
+        assert np.array_equal(hout, _whole_c2c(x, n, b, -1, 5))
