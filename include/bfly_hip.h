@@ -116,12 +116,14 @@ BFLY_API int bfly_rt_time_transform(int kind, int sign, const float* in, float* 
  * bfly_real_ex, BFLY_VARIANT_AUTO meaning the AUTO choice for a chunk's
  * batch, so the result equals bfly_c2c_ex / bfly_real_ex of that variant on
  * the whole batch bit for bit.  `dev_in` / `dev_out` hold the whole batch
- * on the device; `work` is the long-C2C workspace of one chunk
- * (bfly_workspace_bytes(kind, n, chunk rows)) or NULL when it needs none.
- * Host memory may be pinned (bfly_rt_malloc_host) or pageable; pageable
- * buffers (the runtime stages their copies itself, one direction at a time)
- * run the three steps in turn on `stream`, as one chunk.  Calls on one device serialise
- * on the library's streams.  Status codes as for the other entry points
+ * on the device; `work` is the long-C2C workspace of one chunk, 8 n (chunk
+ * rows) bytes, where the chunk's kernel needs one (n > 16384, or the
+ * four-step chain), else NULL.  Host memory may be pinned
+ * (bfly_rt_malloc_host) or pageable; pageable buffers (the runtime stages
+ * their copies itself, one direction at a time) run the three steps in turn
+ * on `stream`: the whole batch at once, or chunk by chunk where the kernel
+ * needs the workspace.  Calls on one device serialise on the library's
+ * streams.  Status codes as for the other entry points
  * (BFLY_ERR_H2D / _LAUNCH / _D2H / _SYNC / _SHAPE); after an error every
  * issued step has finished before the call returns. */
 BFLY_API int bfly_host_pipelined(int kind, int sign, int variant, const float* host_in, float* host_out,
