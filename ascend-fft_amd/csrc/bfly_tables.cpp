@@ -289,13 +289,15 @@ int scaled_variant(uint32_t n, uint32_t batch) {
 }
 
 // Real transforms, once the batch fills several rounds of resident
-// workgroups: streaming with LDS twiddles from n = 1024 (measured, MI355X:
+// workgroups: streaming with LDS twiddles from n = 512 (measured, MI355X:
 // R2C n=8192 b=4096 54.3 us vs 74.1 one-shot / 74.8 streaming; C2R n=4096
-// b=4096 31.2 vs 35.1 / 33.9), the register-table streaming form below
-// (C2R n=256 b=65536 30.6 vs 32.3 LDS twiddles / 34.4 one-shot).
+// b=4096 31.2 vs 35.1 / 33.9; R2C n=512 b=32768 27.2 vs 32.9 / 30.0, C2R
+// 30.0 vs 34.2 / 30.0), the register-table streaming form below (n=256
+// b=65536: R2C 33.4 vs 37.5 LDS twiddles / 35.9 one-shot, C2R 30.2 vs
+// 34.5 / 34.4).
 int auto_real_variant(uint32_t n, uint32_t batch) {
     if ((uint64_t)n * batch < (1ull << 22)) return BFLY_VARIANT_PLAIN;
-    return n >= 1024 ? BFLY_VARIANT_STREAM_LT : BFLY_VARIANT_STREAM;
+    return n >= 512 ? BFLY_VARIANT_STREAM_LT : BFLY_VARIANT_STREAM;
 }
 
 int check_pipelined(int kind, int sign, int variant, const void* host_in, const void* host_out, uint32_t n,
