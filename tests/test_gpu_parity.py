@@ -159,7 +159,7 @@ def test_split_mfma_rejects_other_lengths():
                                   tab.data_ptr(), None, 0) == _lib.ERR_SHAPE
 
 
-@pytest.mark.parametrize("n", [32, 256, 1024, 4096, 8192])
+@pytest.mark.parametrize("n", [32, 64, 128, 256, 1024, 4096, 8192])
 def test_real_streaming_variant_equals_one_shot(n):
     # bfly_real_ex STREAM runs the same arithmetic as PLAIN on a resident
     # grid: outputs equal bit for bit (ragged and multi-round batches,
