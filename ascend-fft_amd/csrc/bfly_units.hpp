@@ -438,8 +438,12 @@ __device__ __forceinline__ void c2r_load(float2 (&q)[bfly_plan::kPoints + 1], co
 }
 
 // The two halves of c2r_finish, for a caller that issues other work (the
-// next rows' loads) between them.  Staging: the thread's bins into its
-// transform's LDS region.
+// next rows' loads) between them, or whose kernel the compiler schedules
+// better through them.  c2r_finish keeps its own single body: built from
+// these two it compiled differently and measured slower for the
+// register-twiddle streaming kernel at 8 threads per transform (256 x 65536
+// C2R 30.2 -> 34 us).  Staging: the thread's bins into its transform's LDS
+// region.
 template <int I, int WG, int SC = 0>
 __device__ __forceinline__ void c2r_stage(const float2 (&q)[bfly_plan::kPoints + 1], float2* lds, int tid) {
     using PL = PlanOf<I, SC>;
